@@ -1,0 +1,254 @@
+"""Python mirror of the convfpga C API (ctypes over lib/libconvfpga.so).
+
+The drop-in functions keep the reference's names, arguments and error
+behaviour (host/include/convfpga/convfpga.h:36-46): fpga_initialize returns
+0 / -1..-5, the conv calls return an fpga_t whose .valid is False on bad
+arguments.  Host buffers are numpy complex64 arrays in natural order
+(index z*N*N + y*N + x).  The extension calls (include/convfpga/convfpga_ext.h)
+take device pointers: anything with a data_ptr() method (a torch tensor on the
+GPU) or a plain integer address.
+
+This module never falls back to a CPU path: if the native library or the code
+object is missing it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libconvfpga.so")
+CODE_OBJECT = os.path.join(_PKG, "lib", "convfpga_gfx950.hsaco")
+
+CF_OK = 0
+CF_ERR_ARG, CF_ERR_SIZE, CF_ERR_STATE, CF_ERR_RUNTIME, CF_ERR_NOMEM = -1, -2, -3, -4, -5
+CF_SCALE_NONE, CF_SCALE_INV_N3 = 0, 1
+CF_PREC_FP32, CF_PREC_SPLIT_FP16 = 0, 1
+CF_MAX_PASSES = 8
+SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
+
+# names of every symbol include/convfpga/*.h declares
+DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpgaf_conv3D",
+                  "fpgaf_conv3D_svm", "fpgaf_conv3D_svm_batch")
+EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
+               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_conv3d_device",
+               "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
+               "cf_profile_end", "cf_fill_uniform")
+
+
+class fpga_t(ctypes.Structure):
+    """Layout of the reference's fpga_t (convfpga.h:24-34)."""
+    _fields_ = [("filter_pcie_wr_t", ctypes.c_double), ("filter_exec_t", ctypes.c_double),
+                ("sig_pcie_wr_t", ctypes.c_double), ("sig_pcie_rd_t", ctypes.c_double),
+                ("sig_exec_t", ctypes.c_double), ("siginv_exec_t", ctypes.c_double),
+                ("valid", ctypes.c_bool)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class cf_pass_times_t(ctypes.Structure):
+    _fields_ = [("npass", ctypes.c_uint), ("ms", ctypes.c_float * CF_MAX_PASSES),
+                ("name", (ctypes.c_char * 24) * CF_MAX_PASSES),
+                ("bytes", ctypes.c_double * CF_MAX_PASSES)]
+
+    def as_list(self):
+        return [{"name": self.name[i].value.decode(), "ms": float(self.ms[i]), "bytes": float(self.bytes[i])}
+                for i in range(self.npass)]
+
+
+class ConvError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libconvfpga.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ConvError("libconvfpga.so not built: run `python -m convfpga_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u, i, sz, d = ctypes.c_void_p, ctypes.c_uint, ctypes.c_int, ctypes.c_size_t, ctypes.c_double
+        sig = {
+            "fpga_initialize": (i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_bool]),
+            "fpga_final": (None, []),
+            "fpgaf_complex_malloc": (vp, [sz]),
+            "fpgaf_conv3D": (fpga_t, [u, vp, vp, vp]),
+            "fpgaf_conv3D_svm": (fpga_t, [u, vp, vp, vp]),
+            "fpgaf_conv3D_svm_batch": (fpga_t, [u, vp, vp, vp, u]),
+            "cf_last_error": (ctypes.c_char_p, []),
+            "cf_supported_size": (i, [u]),
+            "cf_spectrum_create": (i, [u, u, vp, i, vp, ctypes.POINTER(vp)]),
+            "cf_spectrum_destroy": (None, [vp]),
+            "cf_spectrum_data": (vp, [vp]),
+            "cf_spectrum_edge": (u, [vp]),
+            "cf_spectrum_count": (u, [vp]),
+            "cf_conv3d_device": (i, [vp, vp, vp, u, i, i, vp]),
+            "cf_conv3d_device_timed": (i, [vp, vp, vp, u, i, i, vp, ctypes.POINTER(cf_pass_times_t)]),
+            "cf_fft3d_device": (i, [u, vp, vp, u, i, vp]),
+            "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
+            "cf_profile_begin": (i, [u]),
+            "cf_fill_uniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
+            "cf_profile_end": (i, [ctypes.POINTER(cf_pass_times_t), ctypes.POINTER(u)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().cf_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != CF_OK:
+        raise ConvError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _host(a, n_elems=None):
+    if not isinstance(a, np.ndarray) or a.dtype != np.complex64 or not a.flags.c_contiguous:
+        raise TypeError("host buffers must be C-contiguous numpy complex64 arrays")
+    if n_elems is not None and a.size < n_elems:
+        raise ValueError("buffer holds %d values, need %d" % (a.size, n_elems))
+    return a.ctypes.data
+
+
+def _dev(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream or None
+    return stream.cuda_stream or None
+
+
+# ---------------------------------------------------------------- drop-in API
+def fpga_initialize(platform_name="gfx950", path=CODE_OBJECT, use_svm=False):
+    pn = None if platform_name is None else platform_name.encode()
+    pa = None if path is None else path.encode()
+    return lib().fpga_initialize(pn, pa, bool(use_svm))
+
+
+def fpga_final():
+    lib().fpga_final()
+
+
+def fpgaf_complex_malloc(sz):
+    return lib().fpgaf_complex_malloc(sz)
+
+
+def _conv_args(N, sig, filt, out, how_many=1):
+    # sizes are only checked for a power-of-two N; any other N is passed
+    # through so that the library reports it as the reference does (valid=0)
+    n3 = N ** 3 if N > 0 and (N & (N - 1)) == 0 else 0
+    p = []
+    for a, need in ((sig, n3 * how_many), (filt, n3), (out, n3 * how_many)):
+        p.append(None if a is None else _host(a, need))
+    return p
+
+
+def fpgaf_conv3D(N, sig, filt, out):
+    s, f, o = _conv_args(N, sig, filt, out)
+    return lib().fpgaf_conv3D(N, s, f, o)
+
+
+def fpgaf_conv3D_svm(N, sig, filt, out):
+    s, f, o = _conv_args(N, sig, filt, out)
+    return lib().fpgaf_conv3D_svm(N, s, f, o)
+
+
+def fpgaf_conv3D_svm_batch(N, sig, filt, out, how_many):
+    s, f, o = _conv_args(N, sig, filt, out, max(how_many, 1))
+    return lib().fpgaf_conv3D_svm_batch(N, s, f, o, how_many)
+
+
+# -------------------------------------------------------------- extensions
+def supported_size(N):
+    return bool(lib().cf_supported_size(N))
+
+
+class Spectrum:
+    """Precomputed filter spectra (cf_spectrum_t): count filters of edge N."""
+
+    def __init__(self, N, filters, count=1, on_device=True, stream=None):
+        h = ctypes.c_void_p()
+        if on_device:
+            ptr = _dev(filters)
+        else:
+            ptr = _host(filters, N ** 3 * count)
+        _check(lib().cf_spectrum_create(N, count, ptr, 1 if on_device else 0, _stream(stream), ctypes.byref(h)),
+               "cf_spectrum_create")
+        self._h = h
+        self.N, self.count = N, count
+
+    @property
+    def handle(self):
+        return self._h
+
+    def data_ptr(self):
+        return lib().cf_spectrum_data(self._h)
+
+    def close(self):
+        if self._h:
+            lib().cf_spectrum_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def conv3d_device(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
+    _check(lib().cf_conv3d_device(spec.handle, _dev(sig), _dev(out), how_many, scale, precision, _stream(stream)),
+           "cf_conv3d_device")
+
+
+def conv3d_device_timed(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
+    t = cf_pass_times_t()
+    _check(lib().cf_conv3d_device_timed(spec.handle, _dev(sig), _dev(out), how_many, scale, precision,
+                                        _stream(stream), ctypes.byref(t)), "cf_conv3d_device_timed")
+    return t.as_list()
+
+
+def fft3d_device(N, x, out, how_many=1, inverse=False, stream=None):
+    _check(lib().cf_fft3d_device(N, _dev(x), _dev(out), how_many, 1 if inverse else 0, _stream(stream)),
+           "cf_fft3d_device")
+
+
+def rel_l2_device(test, ref, npts, ref_is_f64, stream=None):
+    r = ctypes.c_double()
+    _check(lib().cf_rel_l2_device(_dev(test), _dev(ref), 1 if ref_is_f64 else 0, npts, _stream(stream),
+                                  ctypes.byref(r)), "cf_rel_l2_device")
+    return r.value
+
+
+def profile_begin(max_calls):
+    _check(lib().cf_profile_begin(max_calls), "cf_profile_begin")
+
+
+def profile_end():
+    """-> (list of {name, ms (mean per call), bytes}, number of calls recorded)"""
+    t = cf_pass_times_t()
+    n = ctypes.c_uint()
+    _check(lib().cf_profile_end(ctypes.byref(t), ctypes.byref(n)), "cf_profile_end")
+    return t.as_list(), n.value
+
+
+def fill_uniform(x, nfloats, seed, stream=None):
+    """Device-side U[-1,1) fill, bit-identical to the host splitmix64 generator."""
+    _check(lib().cf_fill_uniform(_dev(x), nfloats, seed, _stream(stream)), "cf_fill_uniform")

@@ -1,0 +1,642 @@
+// engine.cpp -- HIP host runtime of libconvfpga: device selection, the kernel
+// code object, per-size twiddle tables, workspaces, pass scheduling and timing.
+//
+// Replaces the reference's OpenCL host layer (host/src/fpga_init.c,
+// host/src/convfpga*.c, host/src/opencl_utils.c).  Differences by design:
+//  * one in-order HIP stream instead of eight profiling queues per call
+//    (fpga_init.c:142-183): the passes are dependent kernels, not a dataflow
+//    pipeline of concurrently running tasks;
+//  * device buffers, pinned staging and twiddle tables are cached across calls
+//    (the reference creates and frees 4 buffers and 8 kernels per call,
+//    convfpga.c:40-72, 377-405);
+//  * errors come back as return codes (the reference exit()s,
+//    opencl_utils.c:421-439);
+//  * every entry point holds one process-wide lock (the reference keeps global
+//    OpenCL state and is not thread-safe, fpga_state.h:8-17).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "convfpga/convfpga_ext.h"
+#include "engine.h"
+#include "geometry.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(CF_ERR_RUNTIME, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CFCHK(expr)          \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != CF_OK) return r_; \
+  } while (0)
+
+struct SizeFns {
+  hipFunction_t row_fwd = nullptr, row_inv = nullptr, coly_fwd = nullptr, coly_inv = nullptr;
+  hipFunction_t colz_fwd = nullptr, colz_inv = nullptr, colz_conv = nullptr;
+  void* tw = nullptr;  // N float2 twiddles W_N^j
+};
+
+struct Engine {
+  std::recursive_mutex mu;
+  bool ready = false;
+  bool svm = false;
+  int device = 0;
+  hipModule_t mod = nullptr;
+  hipStream_t stream = nullptr;
+  hipFunction_t twgen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr;
+  std::map<unsigned, SizeFns> sizes;
+  // workspace of the host-buffer (drop-in) calls
+  void* d_data = nullptr;
+  size_t d_data_bytes = 0;
+  void* d_filt = nullptr;
+  size_t d_filt_bytes = 0;
+  void* h_sig = nullptr;   // pinned, host-mapped
+  size_t h_sig_bytes = 0;
+  void* h_filt = nullptr;
+  size_t h_filt_bytes = 0;
+  void* h_out = nullptr;
+  size_t h_out_bytes = 0;
+  // error-norm scratch
+  void* d_err = nullptr;
+  size_t d_err_bytes = 0;
+  hipEvent_t ev[CF_MAX_PASSES + 8] = {};
+  // cf_profile_begin/end: events of up to prof_max calls, (CF_MAX_PASSES+1) each
+  std::vector<hipEvent_t> prof_ev;
+  unsigned prof_max = 0, prof_calls = 0, prof_npass = 0;
+  std::vector<std::string> prof_names;
+  std::vector<double> prof_bytes;
+};
+
+Engine g;
+constexpr int kErrBlocks = 1024;
+constexpr int kNumEvents = CF_MAX_PASSES + 8;
+
+size_t cube_elems(unsigned n) { return (size_t)n * n * n; }
+size_t cube_bytes(unsigned n) { return cube_elems(n) * 8; }
+
+std::string lower(const std::string& s) {
+  std::string r = s;
+  for (auto& ch : r) ch = (char)std::tolower((unsigned char)ch);
+  return r;
+}
+
+void release_all() {
+  if (g.ready || g.stream) hipSetDevice(g.device);
+  if (g.stream) hipStreamSynchronize(g.stream);
+  for (auto& kv : g.sizes)
+    if (kv.second.tw) hipFree(kv.second.tw);
+  g.sizes.clear();
+  if (g.d_data) hipFree(g.d_data);
+  if (g.d_filt) hipFree(g.d_filt);
+  if (g.d_err) hipFree(g.d_err);
+  if (g.h_sig) hipHostFree(g.h_sig);
+  if (g.h_filt) hipHostFree(g.h_filt);
+  if (g.h_out) hipHostFree(g.h_out);
+  g.d_data = g.d_filt = g.d_err = g.h_sig = g.h_filt = g.h_out = nullptr;
+  g.d_data_bytes = g.d_filt_bytes = g.d_err_bytes = g.h_sig_bytes = g.h_filt_bytes = g.h_out_bytes = 0;
+  for (auto& e : g.ev)
+    if (e) {
+      hipEventDestroy(e);
+      e = nullptr;
+    }
+  for (auto e : g.prof_ev) hipEventDestroy(e);
+  g.prof_ev.clear();
+  g.prof_max = g.prof_calls = 0;
+  if (g.stream) hipStreamDestroy(g.stream);
+  g.stream = nullptr;
+  if (g.mod) hipModuleUnload(g.mod);
+  g.mod = nullptr;
+  g.twgen = g.errp = g.errf = g.fillu = nullptr;
+  g.ready = false;
+  g.svm = false;
+}
+
+int get_fn(hipFunction_t* f, const char* name) {
+  hipError_t e = hipModuleGetFunction(f, g.mod, name);
+  if (e != hipSuccess) return fail(CF_ERR_RUNTIME, "kernel %s not in code object: %s", name, hipGetErrorString(e));
+  return CF_OK;
+}
+
+int require_ready() {
+  if (!g.ready) return fail(CF_ERR_STATE, "fpga_initialize() has not succeeded");
+  hipError_t e = hipSetDevice(g.device);
+  if (e != hipSuccess) return fail(CF_ERR_RUNTIME, "hipSetDevice: %s", hipGetErrorString(e));
+  return CF_OK;
+}
+
+// Kernels and FP64-generated twiddle table of edge n (created on first use).
+int size_res(unsigned n, SizeFns** out) {
+  if (!cfgeo::supported((int)n)) return fail(CF_ERR_SIZE, "N=%u is not one of 8,16,32,64,128,256,512", n);
+  auto it = g.sizes.find(n);
+  if (it != g.sizes.end()) {
+    *out = &it->second;
+    return CF_OK;
+  }
+  SizeFns s;
+  char name[64];
+  struct {
+    hipFunction_t* f;
+    const char* fmt;
+  } tab[] = {{&s.row_fwd, "cf_row_fwd_%u"},   {&s.row_inv, "cf_row_inv_%u"},   {&s.coly_fwd, "cf_coly_fwd_%u"},
+             {&s.coly_inv, "cf_coly_inv_%u"}, {&s.colz_fwd, "cf_colz_fwd_%u"}, {&s.colz_inv, "cf_colz_inv_%u"},
+             {&s.colz_conv, "cf_colz_conv_%u"}};
+  for (auto& t : tab) {
+    snprintf(name, sizeof name, t.fmt, n);
+    CFCHK(get_fn(t.f, name));
+  }
+  HIPCHK(hipMalloc(&s.tw, (size_t)n * 8));
+  void* tw = s.tw;
+  int nn = (int)n;
+  void* args[] = {&tw, &nn};
+  HIPCHK(hipModuleLaunchKernel(g.twgen, (n + 255) / 256, 1, 1, 256, 1, 1, 0, g.stream, args, nullptr));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  *out = &(g.sizes[n] = s);
+  return CF_OK;
+}
+
+// ---- passes -------------------------------------------------------------------
+struct Pass {
+  hipFunction_t f;
+  unsigned tiles, threads;
+  const void* in;
+  void* out;
+  const void* H;
+  long long hstride;
+  float scale;
+  const char* name;
+  double bytes;  // algorithmic HBM bytes for the whole launch
+};
+
+int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const void* tw) {
+  const size_t ce = cube_elems(n);
+  const unsigned kMaxY = 65535;
+  for (unsigned b0 = 0; b0 < nb; b0 += kMaxY) {
+    const unsigned cnt = std::min(kMaxY, nb - b0);
+    const void* in = (const char*)p.in + b0 * ce * 8;
+    void* out = (char*)p.out + b0 * ce * 8;
+    const void* H = p.H ? (const void*)((const char*)p.H + (size_t)b0 * p.hstride * 8) : nullptr;
+    long long hs = p.hstride;
+    float sc = p.scale;
+    const void* twp = tw;
+    void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc};
+    HIPCHK(hipModuleLaunchKernel(p.f, p.tiles, cnt, 1, p.threads, 1, 1, 0, s, args, nullptr));
+  }
+  return CF_OK;
+}
+
+// forward 3D FFT: x, y, z passes (in may equal out)
+void fwd_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, std::vector<Pass>& v) {
+  const double io = 2.0 * cube_bytes(n) * nb;
+  const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
+  const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+  v.push_back({f.row_fwd, rT, rt, in, out, nullptr, 0, 1.0f, "x_fwd", io});
+  v.push_back({f.coly_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "y_fwd", io});
+  v.push_back({f.colz_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "z_fwd", io});
+}
+
+void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, float scale, std::vector<Pass>& v) {
+  const double io = 2.0 * cube_bytes(n) * nb;
+  const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
+  const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+  v.push_back({f.colz_inv, cT, ct, in, out, nullptr, 0, 1.0f, "z_inv", io});
+  v.push_back({f.coly_inv, cT, ct, out, out, nullptr, 0, 1.0f, "y_inv", io});
+  v.push_back({f.row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv", io});
+}
+
+// convolution: x fwd, y fwd, z fwd*H*z inv, y inv, x inv(+scale).
+// Returns the index of the first inverse-only pass (for the fpga_t split).
+size_t conv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* sig, void* out, const void* H,
+                   long long hstride, float scale, std::vector<Pass>& v) {
+  const double V = (double)cube_bytes(n);
+  const double io = 2.0 * V * nb;
+  const double hbytes = hstride ? V * nb : V;
+  const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
+  const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+  v.push_back({f.row_fwd, rT, rt, sig, out, nullptr, 0, 1.0f, "x_fwd", io});
+  v.push_back({f.coly_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "y_fwd", io});
+  v.push_back({f.colz_conv, cT, ct, out, out, H, hstride, 1.0f, "z_fwd_mul_z_inv", io + hbytes});
+  const size_t split = v.size();
+  v.push_back({f.coly_inv, cT, ct, out, out, nullptr, 0, 1.0f, "y_inv", io});
+  v.push_back({f.row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv", io});
+  return split;
+}
+
+int run_passes(const std::vector<Pass>& v, unsigned n, unsigned nb, hipStream_t s, const void* tw,
+               hipEvent_t* ev_before /* nullable, v.size()+1 events */) {
+  if (ev_before) HIPCHK(hipEventRecord(ev_before[0], s));
+  for (size_t i = 0; i < v.size(); ++i) {
+    CFCHK(launch_pass(v[i], n, nb, s, tw));
+    if (ev_before) HIPCHK(hipEventRecord(ev_before[i + 1], s));
+  }
+  return CF_OK;
+}
+
+int ensure_device(void** p, size_t* have, size_t need) {
+  if (*have >= need) return CF_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  hipError_t e = hipMalloc(p, need);
+  if (e != hipSuccess) return fail(CF_ERR_NOMEM, "hipMalloc(%zu): %s", need, hipGetErrorString(e));
+  *have = need;
+  return CF_OK;
+}
+
+int ensure_pinned(void** p, size_t* have, size_t need) {
+  if (*have >= need) return CF_OK;
+  if (*p) hipHostFree(*p);
+  *p = nullptr;
+  *have = 0;
+  hipError_t e = hipHostMalloc(p, need, hipHostMallocMapped);
+  if (e != hipSuccess) return fail(CF_ERR_NOMEM, "hipHostMalloc(%zu): %s", need, hipGetErrorString(e));
+  *have = need;
+  return CF_OK;
+}
+
+int ensure_events() {
+  for (auto& e : g.ev)
+    if (!e) HIPCHK(hipEventCreate(&e));
+  return CF_OK;
+}
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+  return (double)ms;
+}
+
+void* mapped(void* host) {
+  void* d = nullptr;
+  if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return host;
+  return d;
+}
+
+}  // namespace
+
+// =============================================================================
+// drop-in back end
+// =============================================================================
+extern "C" int cfe_initialize(const char* platform_name, const char* path, int use_svm) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (g.ready || g.stream || g.mod) release_all();
+  // same order of checks as the reference (fpga_init.c:65-112)
+  if (path == nullptr || path[0] == '\0') return -1;
+  if (platform_name == nullptr || platform_name[0] == '\0') return -2;
+  int ndev = 0;
+  const bool have_rt = hipGetDeviceCount(&ndev) == hipSuccess;
+  if (!have_rt) ndev = 0;
+  std::vector<std::string> names = {"AMD HIP", "ROCm"};
+  hipDeviceProp_t prop;
+  if (ndev > 0 && hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+    names.push_back(prop.name);
+    names.push_back(prop.gcnArchName);
+  }
+  const std::string want = lower(platform_name);
+  bool match = false;
+  for (auto& nm : names) match = match || lower(nm).find(want) != std::string::npos;
+  if (!match) return -2;
+  if (ndev < 1) return -3;
+  g.device = 0;
+  if (hipSetDevice(g.device) != hipSuccess) return -3;
+  if (use_svm) {
+    int can_map = 0;
+    if (hipDeviceGetAttribute(&can_map, hipDeviceAttributeCanMapHostMemory, g.device) != hipSuccess || !can_map)
+      return -5;
+    g.svm = true;
+  }
+  if (hipStreamCreate(&g.stream) != hipSuccess) {
+    release_all();
+    return -3;
+  }
+  if (hipModuleLoad(&g.mod, path) != hipSuccess) {
+    g.mod = nullptr;
+    fprintf(stderr, "convfpga: cannot load code object '%s'\n", path);
+    release_all();
+    return -4;
+  }
+  if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
+      get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
+      ensure_events() != CF_OK) {
+    fprintf(stderr, "convfpga: '%s' is not a convfpga code object (%s)\n", path, g_err.c_str());
+    release_all();
+    return -4;
+  }
+  g.ready = true;
+  return 0;
+}
+
+extern "C" void cfe_final(void) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  release_all();
+}
+
+extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, void* out, unsigned how_many, int mode,
+                             cfe_times_t* t) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!sig || !filter || !out || how_many == 0 || !t) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  const size_t V = cube_bytes(n);
+  const size_t SV = V * how_many;
+  memset(t, 0, sizeof *t);
+  hipEvent_t* ev = g.ev;
+  hipStream_t s = g.stream;
+  std::vector<Pass> filt_p, conv_p;
+
+  if (mode == CFE_MODE_COPY) {
+    CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
+    CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
+    // step 1-2: filter H2D and transform (convfpga.c:75-172)
+    HIPCHK(hipEventRecord(ev[0], s));
+    HIPCHK(hipMemcpyAsync(g.d_filt, filter, V, hipMemcpyHostToDevice, s));
+    fwd_passes(*f, n, 1, g.d_filt, g.d_filt, filt_p);
+    CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev + 1));  // ev[1..4]
+    // step 3: signal H2D, forward + product (convfpga.c:174-279)
+    HIPCHK(hipMemcpyAsync(g.d_data, sig, SV, hipMemcpyHostToDevice, s));
+    const size_t split = conv_passes(*f, n, how_many, g.d_data, g.d_data, g.d_filt, 0, 1.0f, conv_p);
+    CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ev + 5));  // ev[5..10]
+    // step 4 done by the last passes; D2H (convfpga.c:365-375)
+    HIPCHK(hipMemcpyAsync(out, g.d_data, SV, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ev[11], s));
+    HIPCHK(hipStreamSynchronize(s));
+    t->filter_pcie_wr_t = ms_between(ev[0], ev[1]);
+    t->filter_exec_t = ms_between(ev[1], ev[4]);
+    t->sig_pcie_wr_t = ms_between(ev[4], ev[5]);
+    t->sig_exec_t = ms_between(ev[5], ev[5 + split]);
+    t->siginv_exec_t = ms_between(ev[5 + split], ev[10]);
+    t->sig_pcie_rd_t = ms_between(ev[10], ev[11]);
+    return CF_OK;
+  }
+  if (mode == CFE_MODE_SVM) {
+    if (!g.svm) return fail(CF_ERR_STATE, "fpga_initialize() was called with use_svm = false");
+    CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
+    CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
+    CFCHK(ensure_pinned(&g.h_filt, &g.h_filt_bytes, V));
+    CFCHK(ensure_pinned(&g.h_sig, &g.h_sig_bytes, SV));
+    CFCHK(ensure_pinned(&g.h_out, &g.h_out_bytes, SV));
+    // host copies into the mapped buffers (the reference's SVM map + memcpy,
+    // convfpga_svm.c:86-103); the kernels then read them over the bus.
+    memcpy(g.h_filt, filter, V);
+    memcpy(g.h_sig, sig, SV);
+    fwd_passes(*f, n, 1, mapped(g.h_filt), g.d_filt, filt_p);
+    CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev));  // ev[0..3]
+    const size_t split = conv_passes(*f, n, how_many, mapped(g.h_sig), g.d_data, g.d_filt, 0, 1.0f, conv_p);
+    conv_p.back().out = mapped(g.h_out);  // last pass writes host memory directly
+    CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ev + 4));  // ev[4..9]
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(out, g.h_out, SV);
+    t->filter_exec_t = ms_between(ev[0], ev[3]);
+    if (how_many == 1) {
+      t->sig_exec_t = ms_between(ev[4], ev[4 + split]);
+      t->siginv_exec_t = ms_between(ev[4 + split], ev[9]);
+    } else {
+      // batch: summed stage time and first-to-last wall span (convfpga_svm_batch.c:664-665)
+      t->sig_exec_t = ms_between(ev[4], ev[4 + split]) + ms_between(ev[4 + split], ev[9]);
+      t->siginv_exec_t = ms_between(ev[4], ev[9]);
+    }
+    return CF_OK;
+  }
+  return fail(CF_ERR_ARG, "unknown mode %d", mode);
+}
+
+// =============================================================================
+// extension API (convfpga_ext.h) -- the only symbols of this file exported
+// from the shared library (it is built with -fvisibility=hidden)
+// =============================================================================
+#pragma GCC visibility push(default)
+struct cf_spectrum_s {
+  unsigned n = 0, count = 0;
+  void* d = nullptr;
+};
+
+extern "C" const char* cf_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cf_supported_size(unsigned n) { return cfgeo::supported((int)n) ? 1 : 0; }
+
+extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filters, int on_device, void* stream,
+                                  cf_spectrum_t* out) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!filters || !out || count == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  *out = nullptr;
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  hipStream_t s = (hipStream_t)stream;
+  auto* h = new cf_spectrum_s;
+  h->n = n;
+  h->count = count;
+  const size_t bytes = cube_bytes(n) * count;
+  hipError_t e = hipMalloc(&h->d, bytes);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(CF_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  const void* src = filters;
+  if (!on_device) {
+    e = hipMemcpyAsync(h->d, filters, bytes, hipMemcpyHostToDevice, s);
+    src = h->d;
+  }
+  std::vector<Pass> p;
+  fwd_passes(*f, n, count, src, h->d, p);
+  int rc = e == hipSuccess ? run_passes(p, n, count, s, f->tw, nullptr)
+                           : fail(CF_ERR_RUNTIME, "hipMemcpyAsync: %s", hipGetErrorString(e));
+  if (rc == CF_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "spectrum setup failed");
+  if (rc != CF_OK) {
+    hipFree(h->d);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CF_OK;
+}
+
+extern "C" void cf_spectrum_destroy(cf_spectrum_t h) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!h) return;
+  if (h->d) {
+    hipSetDevice(g.device);
+    hipFree(h->d);
+  }
+  delete h;
+}
+
+extern "C" const void* cf_spectrum_data(cf_spectrum_t h) { return h ? h->d : nullptr; }
+extern "C" unsigned cf_spectrum_edge(cf_spectrum_t h) { return h ? h->n : 0; }
+extern "C" unsigned cf_spectrum_count(cf_spectrum_t h) { return h ? h->count : 0; }
+
+static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many, int scale_mode,
+                            int precision, void* stream, cf_pass_times_t* times) {
+  if (!h || !d_sig || !d_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  if (h->count != 1 && h->count != how_many)
+    return fail(CF_ERR_ARG, "spectrum holds %u filters; need 1 or how_many (%u)", h->count, how_many);
+  if (scale_mode != CF_SCALE_NONE && scale_mode != CF_SCALE_INV_N3) return fail(CF_ERR_ARG, "bad scale_mode");
+  if (precision != CF_PREC_FP32) return fail(CF_ERR_ARG, "precision %d is not available", precision);
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(h->n, &f));
+  const unsigned n = h->n;
+  const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+  const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
+  std::vector<Pass> p;
+  conv_passes(*f, n, how_many, d_sig, d_out, h->d, hstride, scale, p);
+  hipStream_t s = (hipStream_t)stream;
+  if (!times) {
+    if (g.prof_max > g.prof_calls) {  // armed by cf_profile_begin
+      hipEvent_t* ev = &g.prof_ev[(size_t)g.prof_calls * (CF_MAX_PASSES + 1)];
+      CFCHK(run_passes(p, n, how_many, s, f->tw, ev));
+      if (g.prof_calls == 0) {
+        g.prof_npass = (unsigned)p.size();
+        g.prof_names.clear();
+        g.prof_bytes.clear();
+        for (auto& q : p) {
+          g.prof_names.push_back(q.name);
+          g.prof_bytes.push_back(q.bytes);
+        }
+      }
+      ++g.prof_calls;
+      return CF_OK;
+    }
+    return run_passes(p, n, how_many, s, f->tw, nullptr);
+  }
+  CFCHK(ensure_events());
+  CFCHK(run_passes(p, n, how_many, s, f->tw, g.ev));
+  HIPCHK(hipEventSynchronize(g.ev[p.size()]));
+  memset(times, 0, sizeof *times);
+  times->npass = (unsigned)p.size();
+  for (size_t i = 0; i < p.size(); ++i) {
+    times->ms[i] = (float)ms_between(g.ev[i], g.ev[i + 1]);
+    snprintf(times->name[i], sizeof times->name[i], "%s", p[i].name);
+    times->bytes[i] = p[i].bytes;
+  }
+  return CF_OK;
+}
+
+extern "C" int cf_conv3d_device(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many, int scale_mode,
+                                int precision, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  return conv_device_impl(h, d_sig, d_out, how_many, scale_mode, precision, stream, nullptr);
+}
+
+extern "C" int cf_conv3d_device_timed(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many,
+                                      int scale_mode, int precision, void* stream, cf_pass_times_t* times) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!times) return fail(CF_ERR_ARG, "times is NULL");
+  return conv_device_impl(h, d_sig, d_out, how_many, scale_mode, precision, stream, times);
+}
+
+extern "C" int cf_profile_begin(unsigned max_calls) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (max_calls == 0 || max_calls > 100000) return fail(CF_ERR_ARG, "max_calls must be in 1..100000");
+  CFCHK(require_ready());
+  for (auto e : g.prof_ev) hipEventDestroy(e);
+  g.prof_ev.assign((size_t)max_calls * (CF_MAX_PASSES + 1), nullptr);
+  for (auto& e : g.prof_ev) HIPCHK(hipEventCreate(&e));
+  g.prof_max = max_calls;
+  g.prof_calls = 0;
+  return CF_OK;
+}
+
+extern "C" int cf_profile_end(cf_pass_times_t* avg, unsigned* ncalls) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!avg || !ncalls) return fail(CF_ERR_ARG, "NULL pointer");
+  memset(avg, 0, sizeof *avg);
+  *ncalls = g.prof_calls;
+  int rc = CF_OK;
+  if (g.prof_calls > 0) {
+    const size_t stride = CF_MAX_PASSES + 1;
+    hipEvent_t last = g.prof_ev[(size_t)(g.prof_calls - 1) * stride + g.prof_npass];
+    if (hipEventSynchronize(last) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "event synchronize failed");
+    avg->npass = g.prof_npass;
+    for (unsigned i = 0; i < g.prof_npass && rc == CF_OK; ++i) {
+      double sum = 0.0;
+      for (unsigned c = 0; c < g.prof_calls; ++c)
+        sum += ms_between(g.prof_ev[c * stride + i], g.prof_ev[c * stride + i + 1]);
+      avg->ms[i] = (float)(sum / g.prof_calls);
+      snprintf(avg->name[i], sizeof avg->name[i], "%s", g.prof_names[i].c_str());
+      avg->bytes[i] = g.prof_bytes[i];
+    }
+  }
+  for (auto e : g.prof_ev) hipEventDestroy(e);
+  g.prof_ev.clear();
+  g.prof_max = g.prof_calls = 0;
+  return rc;
+}
+
+extern "C" int cf_fft3d_device(unsigned n, const void* d_in, void* d_out, unsigned how_many, int inverse,
+                               void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!d_in || !d_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  std::vector<Pass> p;
+  if (inverse)
+    inv_passes(*f, n, how_many, d_in, d_out, 1.0f, p);
+  else
+    fwd_passes(*f, n, how_many, d_in, d_out, p);
+  return run_passes(p, n, how_many, (hipStream_t)stream, f->tw, nullptr);
+}
+
+extern "C" int cf_rel_l2_device(const void* d_test, const void* d_ref, int ref_is_f64, size_t npts, void* stream,
+                                double* rel_l2) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!d_test || !d_ref || !rel_l2 || npts == 0) return fail(CF_ERR_ARG, "NULL pointer or zero size");
+  CFCHK(require_ready());
+  CFCHK(ensure_device(&g.d_err, &g.d_err_bytes, (size_t)(2 * kErrBlocks + 2) * sizeof(double)));
+  hipStream_t s = (hipStream_t)stream;
+  int nblk = (int)std::min<size_t>(kErrBlocks, (npts + 255) / 256);
+  const void* t = d_test;
+  const void* r = d_ref;
+  int f64 = ref_is_f64 ? 1 : 0;
+  long long np = (long long)npts;
+  void* part = g.d_err;
+  void* a1[] = {(void*)&t, (void*)&r, (void*)&f64, (void*)&np, (void*)&part};
+  HIPCHK(hipModuleLaunchKernel(g.errp, nblk, 1, 1, 256, 1, 1, 0, s, a1, nullptr));
+  void* res = (char*)g.d_err + (size_t)2 * kErrBlocks * sizeof(double);
+  void* a2[] = {(void*)&part, (void*)&nblk, (void*)&res};
+  HIPCHK(hipModuleLaunchKernel(g.errf, 1, 1, 1, 256, 1, 1, 0, s, a2, nullptr));
+  double host[2] = {0.0, 0.0};
+  HIPCHK(hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *rel_l2 = host[1] > 0.0 ? std::sqrt(host[0] / host[1]) : std::sqrt(host[0]);
+  return CF_OK;
+}
+extern "C" int cf_fill_uniform(void* d_out, size_t nfloats, unsigned long long seed, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!d_out || nfloats == 0) return fail(CF_ERR_ARG, "NULL pointer or zero size");
+  CFCHK(require_ready());
+  long long nf = (long long)nfloats;
+  unsigned long long sd = seed;
+  const unsigned blocks = (unsigned)std::min<size_t>(4096, (nfloats + 255) / 256);
+  void* args[] = {&d_out, &nf, &sd};
+  HIPCHK(hipModuleLaunchKernel(g.fillu, blocks, 1, 1, 256, 1, 1, 0, (hipStream_t)stream, args, nullptr));
+  return CF_OK;
+}
+#pragma GCC visibility pop

@@ -1,0 +1,30 @@
+// geometry.h -- launch geometry of the axis passes, shared by the kernels and
+// the host runtime (plain constexpr C++, no HIP types).
+#pragma once
+
+namespace cfgeo {
+
+// points per thread for each supported line length (radix plan in fft_core.h)
+constexpr int points_per_thread(int n) {
+  return n == 8 ? 8 : n == 16 ? 16 : n == 32 ? 8 : n == 64 ? 8 : n == 128 ? 16 : n == 256 ? 16 : n == 512 ? 8 : 0;
+}
+constexpr int threads_per_line(int n) { return n / points_per_thread(n); }
+
+// y/z passes: C adjacent x-columns times G lines of the third axis per tile
+constexpr int col_c(int n) { return n < 16 ? n : 16; }
+constexpr int col_g(int n) {
+  return (256 / (col_c(n) * threads_per_line(n))) < 1 ? 1
+         : (256 / (col_c(n) * threads_per_line(n))) > n ? n
+                                                         : 256 / (col_c(n) * threads_per_line(n));
+}
+constexpr int col_threads(int n) { return col_c(n) * threads_per_line(n) * col_g(n); }
+constexpr int col_tiles(int n) { return (n / col_c(n)) * (n / col_g(n)); }
+
+// x pass: L whole rows per tile
+constexpr int row_l(int n) { return (256 / threads_per_line(n)) > n * n ? n * n : 256 / threads_per_line(n); }
+constexpr int row_threads(int n) { return row_l(n) * threads_per_line(n); }
+constexpr int row_tiles(int n) { return n * n / row_l(n); }
+
+constexpr bool supported(int n) { return points_per_thread(n) != 0; }
+
+}  // namespace cfgeo

@@ -1,0 +1,97 @@
+/*
+ * convfpga_ext.h -- extensions of the drop-in API (not needed by callers of
+ * convfpga.h).  Plain C ABI: opaque handles, pointers and sizes; device
+ * pointers are HIP device addresses, `stream` is a hipStream_t passed as
+ * void* (NULL = the library's own stream).  All calls require a successful
+ * fpga_initialize() and return 0 or a negative CF_ERR_* code; the message of
+ * the last error on the calling thread is in cf_last_error().
+ *
+ * Data layout is the drop-in one: N^3 interleaved complex float cubes, index
+ * z*N*N + y*N + x, consecutive cubes of a batch N^3 elements apart.
+ */
+#ifndef CONVFPGA_EXT_H
+#define CONVFPGA_EXT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_ERR_ARG (-1)      /* NULL pointer, zero count, bad enum          */
+#define CF_ERR_SIZE (-2)     /* N not in {8,16,32,64,128,256,512}           */
+#define CF_ERR_STATE (-3)    /* fpga_initialize() has not succeeded         */
+#define CF_ERR_RUNTIME (-4)  /* HIP runtime error                           */
+#define CF_ERR_NOMEM (-5)    /* device or host allocation failed            */
+
+/* scale applied in the last inverse pass */
+#define CF_SCALE_NONE 0     /* reference semantics: N^3 * circular convolution */
+#define CF_SCALE_INV_N3 1   /* 1/N^3: the circular convolution itself          */
+
+/* arithmetic of the radix-16 DFT stages */
+#define CF_PREC_FP32 0        /* FP32 FMA butterflies                        */
+#define CF_PREC_SPLIT_FP16 1  /* MFMA complex GEMM, split hi+lo FP16 operands */
+
+const char *cf_last_error(void);
+/* 1 if N is a supported cube edge, else 0 */
+int cf_supported_size(unsigned N);
+
+/* ---- precomputed filter spectra ("filter setup") ---------------------------
+ * The reference recomputes the filter transform in every call
+ * (host/src/convfpga.c:87-172); a spectrum handle computes it once.
+ * `count` filters of edge N, contiguous, on the host or the device. */
+typedef struct cf_spectrum_s *cf_spectrum_t;
+int cf_spectrum_create(unsigned N, unsigned count, const void *filters, int on_device, void *stream,
+                       cf_spectrum_t *out);
+void cf_spectrum_destroy(cf_spectrum_t h);
+/* device address of the spectra (count * N^3 complex floats, natural order) */
+const void *cf_spectrum_data(cf_spectrum_t h);
+unsigned cf_spectrum_edge(cf_spectrum_t h);
+unsigned cf_spectrum_count(cf_spectrum_t h);
+
+/* ---- device-resident convolution --------------------------------------------
+ * how_many cubes d_sig -> d_out (may be the same buffer).  With a spectrum of
+ * count 1 every cube uses it; with count == how_many cube b uses spectrum b. */
+int cf_conv3d_device(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned how_many, int scale_mode,
+                     int precision, void *stream);
+
+/* Per-pass device times (milliseconds) of one cf_conv3d_device call, measured
+ * with HIP events on the launch stream; blocks until the call has finished. */
+#define CF_MAX_PASSES 8
+typedef struct {
+  unsigned npass;
+  float ms[CF_MAX_PASSES];
+  char name[CF_MAX_PASSES][24];
+  double bytes[CF_MAX_PASSES]; /* algorithmic HBM bytes of each pass */
+} cf_pass_times_t;
+int cf_conv3d_device_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned how_many, int scale_mode,
+                           int precision, void *stream, cf_pass_times_t *times);
+
+/* Event profiling of the next cf_conv3d_device calls without a host sync per
+ * call: cf_profile_begin arms it for up to max_calls calls, cf_profile_end
+ * waits for them and returns each pass's mean device time over the recorded
+ * calls (times->ms) and the number of calls recorded. */
+int cf_profile_begin(unsigned max_calls);
+int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
+
+/* ---- plain 3D transforms (device) ------------------------------------------ */
+/* forward (inverse = 0) or unnormalised inverse (inverse = 1) 3D FFT */
+int cf_fft3d_device(unsigned N, const void *d_in, void *d_out, unsigned how_many, int inverse, void *stream);
+
+/* ---- FP64 error norm on the device -----------------------------------------
+ * rel_l2 = sqrt(sum|test-ref|^2 / sum|ref|^2) accumulated in double; ref is
+ * complex double (ref_is_f64 = 1) or complex float. Blocks until done. */
+int cf_rel_l2_device(const void *d_test, const void *d_ref, int ref_is_f64, size_t npts, void *stream,
+                     double *rel_l2);
+
+/* ---- synthetic inputs --------------------------------------------------------
+ * nfloats floats uniform on [-1, 1) from the portable splitmix64 generator
+ * (value i = k/2^23 - 1, k = top 24 bits of splitmix64(seed + (i+1)*0x9E3779B97F4A7C15)),
+ * written on the device; asynchronous on `stream`. */
+int cf_fill_uniform(void *d_out, size_t nfloats, unsigned long long seed, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""CPU-only checks of the C-ABI boundary: the library loads, exports exactly
+what include/convfpga/*.h declares, keeps the reference's type layouts, and
+validates arguments before touching the GPU (host/src/convfpga.c:33-37,
+host/src/fpga_init.c:37-43, 57-122).  No kernel is launched here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import convfpga_amd as C
+from convfpga_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", "convfpga", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b((?:fpga|fpgaf|cf)_[a-z0-9_A-Z]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _declared("convfpga.h") | _declared("convfpga_ext.h")
+    out = subprocess.check_output(["nm", "-D", "--defined-only", api.LIB_PATH], text=True)
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert declared == exported, (declared ^ exported)
+    assert set(api.DROPIN_SYMBOLS) | set(api.EXT_SYMBOLS) == declared
+    for name in declared:
+        assert hasattr(api.lib(), name)
+
+
+def test_code_object_built_for_gfx950():
+    assert os.path.exists(api.CODE_OBJECT)
+    blob = open(api.CODE_OBJECT, "rb").read()
+    assert b"gfx950" in blob
+    for n in C.SUPPORTED_SIZES:
+        for k in ("row_fwd", "row_inv", "coly_fwd", "coly_inv", "colz_fwd", "colz_inv", "colz_conv"):
+            assert ("cf_%s_%d" % (k, n)).encode() in blob
+
+
+def test_fpga_t_layout():
+    assert ctypes.sizeof(api.fpga_t) == 56
+    assert api.fpga_t.valid.offset == 48
+
+
+def test_initialize_error_codes_without_device():
+    assert C.fpga_initialize("gfx950", None) == -1
+    assert C.fpga_initialize("gfx950", "") == -1
+    assert C.fpga_initialize(None, "x.hsaco") == -2
+    assert C.fpga_initialize("", "x.hsaco") == -2
+    assert C.fpga_initialize("Intel(R) FPGA SDK for OpenCL(TM)", "x.hsaco") == -2
+
+
+def test_final_is_idempotent():
+    C.fpga_final()
+    C.fpga_final()
+
+
+def test_complex_malloc():
+    assert C.fpgaf_complex_malloc(0) is None
+    libc = ctypes.CDLL("libc.so.6")
+    libc.free.argtypes = [ctypes.c_void_p]
+    for sz in (1, 8, 4096, 1 << 20):
+        p = C.fpgaf_complex_malloc(sz)
+        assert p is not None and p % 64 == 0
+        libc.free(p)  # the caller frees with free() (src/main.cpp:90-92)
+
+
+@pytest.mark.parametrize("fn", ["fpgaf_conv3D", "fpgaf_conv3D_svm"])
+def test_conv_rejects_bad_arguments(fn):
+    f = getattr(C, fn)
+    a = np.zeros(8 ** 3, np.complex64)
+    for args in ((8, None, a, a), (8, a, None, a), (8, a, a, None), (6, a, a, a), (0, a, a, a), (12, a, a, a)):
+        r = f(*args)
+        assert r.valid is False
+        assert all(v == 0.0 for k, v in r.as_dict().items() if k != "valid")
+    # power of two but not built (and library not initialised) -> invalid, no crash
+    big = np.zeros(4 ** 3, np.complex64)
+    assert f(4, big, big, big).valid is False
+    assert f(8, a, a, a).valid is False  # not initialised
+
+
+def test_batch_rejects_zero_count():
+    a = np.zeros(8 ** 3, np.complex64)
+    assert C.fpgaf_conv3D_svm_batch(8, a, a, a, 0).valid is False
+    assert C.fpgaf_conv3D_svm_batch(8, a, a, a, 1).valid is False  # not initialised
+
+
+def test_ext_api_reports_state_errors():
+    L = api.lib()
+    assert L.cf_supported_size(256) == 1 and L.cf_supported_size(1024) == 0 and L.cf_supported_size(4) == 0
+    h = ctypes.c_void_p()
+    buf = np.zeros(8 ** 3, np.complex64)
+    rc = L.cf_spectrum_create(8, 1, buf.ctypes.data, 0, None, ctypes.byref(h))
+    assert rc == api.CF_ERR_STATE and "fpga_initialize" in api.last_error()
+    assert L.cf_spectrum_create(8, 0, buf.ctypes.data, 0, None, ctypes.byref(h)) == api.CF_ERR_ARG
+    assert L.cf_conv3d_device(None, None, None, 1, 0, 0, None) == api.CF_ERR_ARG
+    r = ctypes.c_double()
+    assert L.cf_rel_l2_device(None, None, 0, 10, None, ctypes.byref(r)) == api.CF_ERR_ARG
+    assert L.cf_fft3d_device(8, buf.ctypes.data, buf.ctypes.data, 1, 0, None) == api.CF_ERR_STATE
+
+
+def test_host_buffer_type_checks():
+    with pytest.raises(TypeError):
+        C.fpgaf_conv3D(8, np.zeros(512, np.complex128), np.zeros(512, np.complex64), np.zeros(512, np.complex64))
+    with pytest.raises(ValueError):
+        C.fpgaf_conv3D(8, np.zeros(100, np.complex64), np.zeros(512, np.complex64), np.zeros(512, np.complex64))

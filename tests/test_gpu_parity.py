@@ -1,0 +1,210 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars (written here, per test): rel-L2 <= 1e-5 against the FP64 oracle for FP32
+results; SNR > 120 dB against the FP32 restatement of the reference's
+verifier (src/fft_conv3D.cpp:235-258) for the drop-in calls; closed-form KATs
+at the full sizes (256^3, 512^3) where the oracle would be slow.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_FP32 = 1e-5   # FP32 bar against the FP64 oracle
+SNR_DB_MIN = 120.0   # the reference verifier's threshold
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _np(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 128])
+def test_dropin_conv_matches_oracle(gpu, oracle, n):
+    print("case fpgaf_conv3D N=%d" % n, flush=True)
+    sig, filt = oracle.uniform(n ** 3, 1000 + n), oracle.uniform(n ** 3, 2000 + n)
+    out = np.zeros_like(sig)
+    r = gpu.fpgaf_conv3D(n, sig, filt, out)
+    assert r.valid
+    assert r.sig_exec_t > 0 and r.siginv_exec_t > 0 and r.filter_exec_t > 0
+    ref64 = oracle.conv3d_f64(n, sig, filt)
+    assert oracle.rel_l2(out, ref64) <= REL_L2_FP32
+    ref32 = oracle.conv3d_f32(n, sig, filt)
+    assert oracle.snr_db(ref32, out) > SNR_DB_MIN
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_dropin_svm_matches_oracle(gpu, oracle, n):
+    print("case fpgaf_conv3D_svm N=%d" % n, flush=True)
+    sig, filt = oracle.uniform(n ** 3, 3000 + n), oracle.uniform(n ** 3, 4000 + n)
+    out = np.zeros_like(sig)
+    r = gpu.fpgaf_conv3D_svm(n, sig, filt, out)
+    assert r.valid and r.sig_pcie_wr_t == 0.0 and r.sig_pcie_rd_t == 0.0
+    assert oracle.rel_l2(out, oracle.conv3d_f64(n, sig, filt)) <= REL_L2_FP32
+
+
+@pytest.mark.parametrize("how_many", [1, 2, 3])
+def test_dropin_svm_batch_distinct_instances(gpu, oracle, how_many):
+    n = 32
+    print("case fpgaf_conv3D_svm_batch N=%d how_many=%d" % (n, how_many), flush=True)
+    sig = oracle.uniform(how_many * n ** 3, 5000 + how_many)   # every instance different
+    filt = oracle.uniform(n ** 3, 6000)
+    out = np.zeros_like(sig)
+    r = gpu.fpgaf_conv3D_svm_batch(n, sig, filt, out, how_many)
+    assert r.valid
+    ref32 = oracle.conv3d_f32(n, sig, filt, batch=how_many)
+    assert oracle.snr_db(ref32, out) > SNR_DB_MIN
+    for b in range(how_many):
+        s = slice(b * n ** 3, (b + 1) * n ** 3)
+        ref = oracle.conv3d_f64(n, np.ascontiguousarray(sig[s]), filt)
+        assert oracle.rel_l2(np.ascontiguousarray(out[s]), ref) <= REL_L2_FP32
+
+
+def test_dropin_reference_generator(gpu, oracle):
+    """The reference's own inputs (rand()/RAND_MAX, helper.cpp:148-172) at its default N=64."""
+    n = 64
+    print("case ref01 N=64", flush=True)
+    filt = oracle.ref01(n ** 3)
+    sig = oracle.ref01(n ** 3)
+    out = np.zeros_like(sig)
+    assert gpu.fpgaf_conv3D(n, sig, filt, out).valid
+    assert oracle.snr_db(oracle.conv3d_f32(n, sig, filt), out) > SNR_DB_MIN
+    assert oracle.rel_l2(out, oracle.conv3d_f64(n, sig, filt)) <= REL_L2_FP32
+
+
+def test_dropin_repeated_calls_and_sizes(gpu, oracle):
+    """Workspace reuse across calls of different N (the reference rebuilds per call)."""
+    for n in (64, 16, 128, 32):
+        print("case repeat N=%d" % n, flush=True)
+        sig, filt = oracle.uniform(n ** 3, 7000 + n), oracle.uniform(n ** 3, 8000 + n)
+        out = np.zeros_like(sig)
+        assert gpu.fpgaf_conv3D(n, sig, filt, out).valid
+        assert oracle.rel_l2(out, oracle.conv3d_f64(n, sig, filt)) <= REL_L2_FP32
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 128, 256])
+def test_fft3d_device_matches_numpy(gpu, oracle, n):
+    print("case fft3d N=%d" % n, flush=True)
+    x = oracle.uniform(n ** 3, 9000 + n)
+    d = _t(x)
+    y = d.clone()
+    gpu.fft3d_device(n, d, y)
+    ref = oracle.fft3d_f64(x, -1)
+    got = _np(y)
+    assert oracle.rel_l2(got, ref) <= REL_L2_FP32
+    z = y.clone()
+    gpu.fft3d_device(n, y, z, inverse=True)
+    back = _np(z)
+    assert oracle.rel_l2(back, x.astype(np.complex128) * n ** 3) <= REL_L2_FP32
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_device_conv_normalised(gpu, oracle, n):
+    print("case device conv scale=1/N^3 N=%d" % n, flush=True)
+    sig, filt = oracle.uniform(n ** 3, 11000 + n), oracle.uniform(n ** 3, 12000 + n)
+    spec = gpu.Spectrum(n, _t(filt))
+    ds = _t(sig)
+    do = ds.clone().zero_()
+    gpu.conv3d_device(spec, ds, do)
+    ref = oracle.conv3d_f64(n, sig, filt, scale=1.0 / n ** 3)
+    assert oracle.rel_l2(_np(do), ref) <= REL_L2_FP32
+    # in place gives the same bits
+    gpu.conv3d_device(spec, ds, ds)
+    assert np.array_equal(_np(ds), _np(do))
+
+
+def test_device_conv_loguniform(gpu, oracle):
+    n = 64
+    print("case loguniform N=64", flush=True)
+    sig, filt = oracle.loguniform(n ** 3, 13001), oracle.loguniform(n ** 3, 13002)
+    spec = gpu.Spectrum(n, _t(filt))
+    ds = _t(sig)
+    gpu.conv3d_device(spec, ds, ds)
+    assert oracle.rel_l2(_np(ds), oracle.conv3d_f64(n, sig, filt, scale=1.0 / n ** 3)) <= REL_L2_FP32
+
+
+def test_device_batch_shared_and_own_spectra(gpu, oracle):
+    n, b = 64, 4
+    print("case batch shared/own N=64 b=4", flush=True)
+    sig = oracle.uniform(b * n ** 3, 14000)
+    filts = oracle.uniform(b * n ** 3, 15000)
+    ds = _t(sig)
+    shared = gpu.Spectrum(n, _t(filts[:n ** 3]))
+    own = gpu.Spectrum(n, _t(filts), count=b)
+    o1, o2 = ds.clone(), ds.clone()
+    gpu.conv3d_device(shared, ds, o1, how_many=b)
+    gpu.conv3d_device(own, ds, o2, how_many=b)
+    g1, g2 = _np(o1), _np(o2)
+    for i in range(b):
+        s = slice(i * n ** 3, (i + 1) * n ** 3)
+        si = np.ascontiguousarray(sig[s])
+        r1 = oracle.conv3d_f64(n, si, np.ascontiguousarray(filts[:n ** 3]), 1.0 / n ** 3)
+        r2 = oracle.conv3d_f64(n, si, np.ascontiguousarray(filts[s]), 1.0 / n ** 3)
+        assert oracle.rel_l2(np.ascontiguousarray(g1[s]), r1) <= REL_L2_FP32
+        assert oracle.rel_l2(np.ascontiguousarray(g2[s]), r2) <= REL_L2_FP32
+
+
+def test_256_conv_matches_oracle(gpu, oracle):
+    n = 256
+    print("case 256^3 conv vs FP64 oracle", flush=True)
+    sig, filt = oracle.uniform(n ** 3, 16001), oracle.uniform(n ** 3, 16002)
+    spec = gpu.Spectrum(n, _t(filt))
+    ds = _t(sig)
+    do = ds.clone()
+    gpu.conv3d_device(spec, ds, do)
+    ref = oracle.conv3d_f64(n, sig, filt, scale=1.0 / n ** 3)
+    got = _np(do)
+    err = oracle.rel_l2(got, ref)
+    assert err <= REL_L2_FP32
+    # the FP64 device error norm agrees with the host one
+    dref = _t(ref)
+    dev_err = gpu.rel_l2_device(do, dref, n ** 3, ref_is_f64=True)
+    assert abs(dev_err - err) <= 1e-3 * err
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_full_size_delta_shift_and_roundtrip(gpu, oracle, n):
+    import torch
+    print("case KAT delta-shift / round trip N=%d" % n, flush=True)
+    sig = oracle.uniform(n ** 3, 17000 + n)
+    ds = _t(sig)
+    d = torch.zeros_like(ds)
+    a, b, c = 5, n - 3, 17
+    d.view(torch.float32)[2 * ((c * n + b) * n + a)] = 1.0
+    spec = gpu.Spectrum(n, d)
+    out = torch.empty_like(ds)
+    gpu.conv3d_device(spec, ds, out)
+    want = torch.roll(ds.view(n, n, n), shifts=(c, b, a), dims=(0, 1, 2)).reshape(-1)
+    err = gpu.rel_l2_device(out, want, n ** 3, ref_is_f64=False)
+    assert err <= REL_L2_FP32
+    y = torch.empty_like(ds)
+    gpu.fft3d_device(n, ds, y)
+    gpu.fft3d_device(n, y, y, inverse=True)
+    y.mul_(1.0 / n ** 3)
+    assert gpu.rel_l2_device(y, ds, n ** 3, ref_is_f64=False) <= REL_L2_FP32
+    del ds, d, out, y, want
+    torch.cuda.empty_cache()
+
+
+def test_rel_l2_device_matches_numpy(gpu, oracle):
+    a = oracle.uniform(100003, 1)
+    b = (a + np.complex64(1e-3) * oracle.uniform(100003, 2)).astype(np.complex64)
+    want = np.linalg.norm(a.astype(np.complex128) - b) / np.linalg.norm(b.astype(np.complex128))
+    got = gpu.rel_l2_device(_t(a), _t(b), a.size, ref_is_f64=False)
+    assert abs(got - want) <= 1e-12 * want
+    got64 = gpu.rel_l2_device(_t(a), _t(b.astype(np.complex128)), a.size, ref_is_f64=True)
+    assert abs(got64 - want) <= 1e-12 * want
+
+
+def test_device_fill_matches_host_generator(gpu, oracle):
+    import torch
+    x = torch.empty(100001, dtype=torch.complex64, device="cuda")
+    gpu.api.fill_uniform(x, 2 * x.numel(), 987654321)
+    want = oracle.uniform(x.numel(), 987654321)
+    assert np.array_equal(_np(x).view(np.uint32), want.view(np.uint32))
