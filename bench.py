@@ -85,6 +85,23 @@ def cpu_baseline_leg(n, batch, gpu_out_host, seed_sig, seed_filt, reps):
     return res, check
 
 
+def copy_bandwidth(torch, nelem, reps=5):
+    """GB/s of torch's device copy of nelem complex64 values (read + write)."""
+    a = torch.empty(nelem, dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    a.zero_()
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * 8 * nelem / (ms * 1e-3) / 1e9
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -110,6 +127,10 @@ def main():
     torch.cuda.synchronize()
     spec = C.Spectrum(n, filt)  # filter setup, outside the timed region
     out = torch.empty_like(sig)
+
+    # plain copy kernel of the same footprint: measured HBM bandwidth beside the
+    # datasheet peak, and the calibration dispatch for the HBM byte counters
+    copy_gbs = copy_bandwidth(torch, max(batch * P, 1 << 24))
 
     for _ in range(a.warmup):
         C.conv3d_device(spec, sig, out, how_many=batch)
@@ -154,6 +175,7 @@ def main():
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
         "conv_roofline_frac": alg_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS,
         "device_ms_per_step": dev_ms,
+        "copy_kernel_GBps": copy_gbs,
         "passes": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }
     if not a.no_cpu_baseline:
