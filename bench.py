@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--workload", default="c2_256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--plan", type=int, default=0, help="force a schedule (CF_PLAN_*); 0 = autotuned")
     return ap.parse_args()
 
 
@@ -87,9 +88,10 @@ def cpu_baseline_leg(n, batch, gpu_out_host, seed_sig, seed_filt, reps):
 
 def copy_bandwidth(torch, nelem, reps=5):
     """GB/s of torch's device copy of nelem complex64 values (read + write)."""
+    from convfpga_amd import api
     a = torch.empty(nelem, dtype=torch.complex64, device="cuda")
     b = torch.empty_like(a)
-    a.zero_()
+    api.fill_uniform(a, 2 * nelem, 99)  # random data: the memory system is faster on zeros
     b.copy_(a)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -125,7 +127,12 @@ def main():
     api.fill_uniform(sig, 2 * P * batch, seed_sig)  # one stream over the batch: every cube differs
     api.fill_uniform(filt, 2 * P, seed_filt)
     torch.cuda.synchronize()
+    plan, tune_ms = api.autotune(n)  # times every supported schedule once, keeps the fastest
+    if a.plan:
+        api.set_plan(n, a.plan)
+        plan = a.plan
     spec = C.Spectrum(n, filt)  # filter setup, outside the timed region
+    assert spec.plan == plan
     out = torch.empty_like(sig)
 
     # plain copy kernel of the same footprint: measured HBM bandwidth beside the
@@ -169,13 +176,17 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded U[-1,1) re/im generated on the GPU)",
         "config": {"workload": "conv3d_%d^3_c2c_fp32_batch%d_precomputed_filter_spectrum" % (n, batch),
-                   "N": n, "batch": batch, "passes": len(passes), "scale": "1/N^3",
+                   "N": n, "batch": batch, "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
                    "algorithmic_bytes_per_step": alg_bytes},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
         "conv_roofline_frac": alg_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS,
+        # the same rate priced with the 5-pass model (11 volume transfers per
+        # convolution with its own spectrum, 2*5 per batched cube + 1)
+        "conv_roofline_frac_5pass_model": (2 * 5 * batch + 1) * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS,
         "device_ms_per_step": dev_ms,
         "copy_kernel_GBps": copy_gbs,
+        "autotune_ms_per_conv": {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()},
         "passes": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
     }
     if not a.no_cpu_baseline:

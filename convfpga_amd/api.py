@@ -25,6 +25,10 @@ CF_ERR_ARG, CF_ERR_SIZE, CF_ERR_STATE, CF_ERR_RUNTIME, CF_ERR_NOMEM = -1, -2, -3
 CF_SCALE_NONE, CF_SCALE_INV_N3 = 0, 1
 CF_PREC_FP32, CF_PREC_SPLIT_FP16 = 0, 1
 CF_MAX_PASSES = 8
+CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
+CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
+CF_NUM_PLANS = 7
+PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
 # names of every symbol include/convfpga/*.h declares
@@ -33,7 +37,8 @@ DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpga
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
                "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_conv3d_device",
                "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
-               "cf_profile_end", "cf_fill_uniform")
+               "cf_profile_end", "cf_fill_uniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
+               "cf_autotune", "cf_spectrum_plan")
 
 
 class fpga_t(ctypes.Structure):
@@ -92,6 +97,11 @@ def lib():
             "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
             "cf_profile_begin": (i, [u]),
             "cf_fill_uniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
+            "cf_plan_supported": (i, [u, i]),
+            "cf_set_plan": (i, [u, i]),
+            "cf_get_plan": (i, [u]),
+            "cf_autotune": (i, [u, ctypes.POINTER(ctypes.c_float)]),
+            "cf_spectrum_plan": (i, [vp]),
             "cf_profile_end": (i, [ctypes.POINTER(cf_pass_times_t), ctypes.POINTER(u)]),
         }
         for name, (res, args) in sig.items():
@@ -201,6 +211,10 @@ class Spectrum:
     def data_ptr(self):
         return lib().cf_spectrum_data(self._h)
 
+    @property
+    def plan(self):
+        return lib().cf_spectrum_plan(self._h)
+
     def close(self):
         if self._h:
             lib().cf_spectrum_destroy(self._h)
@@ -252,3 +266,27 @@ def profile_end():
 def fill_uniform(x, nfloats, seed, stream=None):
     """Device-side U[-1,1) fill, bit-identical to the host splitmix64 generator."""
     _check(lib().cf_fill_uniform(_dev(x), nfloats, seed, _stream(stream)), "cf_fill_uniform")
+
+
+def plan_supported(N, plan):
+    return bool(lib().cf_plan_supported(N, plan))
+
+
+def set_plan(N, plan):
+    _check(lib().cf_set_plan(N, plan), "cf_set_plan")
+
+
+def get_plan(N):
+    rc = lib().cf_get_plan(N)
+    if rc < 0:
+        _check(rc, "cf_get_plan")
+    return rc
+
+
+def autotune(N):
+    """-> (chosen plan, {plan: mean ms per convolution})"""
+    ms = (ctypes.c_float * CF_NUM_PLANS)()
+    rc = lib().cf_autotune(N, ms)
+    if rc < 0:
+        _check(rc, "cf_autotune")
+    return rc, {p: ms[p] for p in range(1, CF_NUM_PLANS) if ms[p] > 0}

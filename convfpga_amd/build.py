@@ -41,12 +41,14 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, extra_hip_flags=(), hsaco=None):
     os.makedirs(LIBDIR, exist_ok=True)
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    tmp = HSACO + ".tmp"
-    _run([hipcc, "--genco"] + HIP_FLAGS + ["-I" + os.path.join(CSRC, "kernels")] + KERNEL_SOURCES + ["-o", tmp])
-    os.replace(tmp, HSACO)
+    target = hsaco or HSACO
+    tmp = target + ".tmp"
+    _run([hipcc, "--genco"] + HIP_FLAGS + list(extra_hip_flags) + ["-I" + os.path.join(CSRC, "kernels")]
+         + KERNEL_SOURCES + ["-o", tmp])
+    os.replace(tmp, target)
     cxx = os.environ.get("CXX", "g++")
     tmp = SHLIB + ".tmp"
     _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter",

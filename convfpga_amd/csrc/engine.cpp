@@ -59,8 +59,24 @@ int fail(int code, const char* fmt, ...) {
 struct SizeFns {
   hipFunction_t row_fwd = nullptr, row_inv = nullptr, coly_fwd = nullptr, coly_inv = nullptr;
   hipFunction_t colz_fwd = nullptr, colz_inv = nullptr, colz_conv = nullptr;
+  hipFunction_t plane_fwd = nullptr, plane_inv = nullptr;                 // N in {32, 64, 128}
+  hipFunction_t xy1_fwd[4] = {}, xy1_inv[4] = {}, mid[4] = {}, hlay[4] = {};  // N = 256
   void* tw = nullptr;  // N float2 twiddles W_N^j
+  int forced = CF_PLAN_AUTO;  // cf_set_plan
+  int tuned = CF_PLAN_AUTO;   // autotuner's choice, CF_PLAN_AUTO until it has run
+  float tune_ms[CF_NUM_PLANS] = {};
 };
+
+bool is_split(int plan) { return plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G44; }
+int split_variant(int plan) { return plan - CF_PLAN_SPLIT3_G22; }
+
+bool plan_ok(unsigned n, int plan) {
+  if (!cfgeo::supported((int)n)) return false;
+  if (plan == CF_PLAN_AXIS5) return true;
+  if (plan == CF_PLAN_PLANE3) return cfgeo::plane_supported((int)n);
+  if (is_split(plan)) return n == 256;
+  return false;
+}
 
 struct Engine {
   std::recursive_mutex mu;
@@ -82,6 +98,13 @@ struct Engine {
   size_t h_filt_bytes = 0;
   void* h_out = nullptr;
   size_t h_out_bytes = 0;
+  // workspace of the 3-pass four-step plans, ordered across streams by work_ev
+  void* d_work = nullptr;
+  size_t d_work_bytes = 0;
+  hipStream_t work_stream = nullptr;
+  bool work_used = false;
+  hipEvent_t work_ev = nullptr;
+  std::string arch;
   // error-norm scratch
   void* d_err = nullptr;
   size_t d_err_bytes = 0;
@@ -115,6 +138,13 @@ void release_all() {
   if (g.d_data) hipFree(g.d_data);
   if (g.d_filt) hipFree(g.d_filt);
   if (g.d_err) hipFree(g.d_err);
+  if (g.d_work) hipFree(g.d_work);
+  g.d_work = nullptr;
+  g.d_work_bytes = 0;
+  g.work_used = false;
+  g.work_stream = nullptr;
+  if (g.work_ev) hipEventDestroy(g.work_ev);
+  g.work_ev = nullptr;
   if (g.h_sig) hipHostFree(g.h_sig);
   if (g.h_filt) hipHostFree(g.h_filt);
   if (g.h_out) hipHostFree(g.h_out);
@@ -150,6 +180,34 @@ int require_ready() {
   return CF_OK;
 }
 
+// Autotuner cache: lines "<arch> <N> <plan>" in $CONVFPGA_TUNE_FILE (last one wins).
+void read_tune_file(unsigned n, SizeFns* s) {
+  const char* path = getenv("CONVFPGA_TUNE_FILE");
+  if (!path || !*path) return;
+  FILE* fp = fopen(path, "r");
+  if (!fp) return;
+  char arch[128];
+  unsigned nn = 0;
+  int plan = 0;
+  char line[512];
+  while (fgets(line, sizeof line, fp)) {
+    if (sscanf(line, "%127s %u %d", arch, &nn, &plan) == 3 && nn == n && g.arch == arch && plan_ok(n, plan))
+      s->tuned = plan;
+  }
+  fclose(fp);
+}
+
+void write_tune_file(unsigned n, const SizeFns& s) {
+  const char* path = getenv("CONVFPGA_TUNE_FILE");
+  if (!path || !*path) return;
+  FILE* fp = fopen(path, "a");
+  if (!fp) return;
+  fprintf(fp, "%s %u %d", g.arch.c_str(), n, s.tuned);
+  for (int p = 1; p < CF_NUM_PLANS; ++p) fprintf(fp, " %.4f", s.tune_ms[p]);
+  fprintf(fp, "\n");
+  fclose(fp);
+}
+
 // Kernels and FP64-generated twiddle table of edge n (created on first use).
 int size_res(unsigned n, SizeFns** out) {
   if (!cfgeo::supported((int)n)) return fail(CF_ERR_SIZE, "N=%u is not one of 8,16,32,64,128,256,512", n);
@@ -170,6 +228,26 @@ int size_res(unsigned n, SizeFns** out) {
     snprintf(name, sizeof name, t.fmt, n);
     CFCHK(get_fn(t.f, name));
   }
+  if (cfgeo::plane_supported((int)n)) {
+    snprintf(name, sizeof name, "cf_plane_fwd_%u", n);
+    CFCHK(get_fn(&s.plane_fwd, name));
+    snprintf(name, sizeof name, "cf_plane_inv_%u", n);
+    CFCHK(get_fn(&s.plane_inv, name));
+  }
+  if (n == 256) {
+    for (int vi = 0; vi < 4; ++vi) {
+      const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi);
+      snprintf(name, sizeof name, "cf_xy1_fwd_256_g%d%d", ga, gb);
+      CFCHK(get_fn(&s.xy1_fwd[vi], name));
+      snprintf(name, sizeof name, "cf_xy1_inv_256_g%d%d", ga, gb);
+      CFCHK(get_fn(&s.xy1_inv[vi], name));
+      snprintf(name, sizeof name, "cf_mid_256_g%d%d", ga, gb);
+      CFCHK(get_fn(&s.mid[vi], name));
+      snprintf(name, sizeof name, "cf_hlayout_256_g%d%d", ga, gb);
+      CFCHK(get_fn(&s.hlay[vi], name));
+    }
+  }
+  read_tune_file(n, &s);
   HIPCHK(hipMalloc(&s.tw, (size_t)n * 8));
   void* tw = s.tw;
   int nn = (int)n;
@@ -183,7 +261,7 @@ int size_res(unsigned n, SizeFns** out) {
 // ---- passes -------------------------------------------------------------------
 struct Pass {
   hipFunction_t f;
-  unsigned tiles, threads;
+  unsigned tiles, threads;  // tiles per cube (tiled passes) or blocks per cube (others)
   const void* in;
   void* out;
   const void* H;
@@ -191,21 +269,29 @@ struct Pass {
   float scale;
   const char* name;
   double bytes;  // algorithmic HBM bytes for the whole launch
+  bool tiled = true;
 };
 
 int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const void* tw) {
   const size_t ce = cube_elems(n);
-  const unsigned kMaxY = 65535;
-  for (unsigned b0 = 0; b0 < nb; b0 += kMaxY) {
-    const unsigned cnt = std::min(kMaxY, nb - b0);
+  // tiled passes cover the whole batch with one 1-D grid (one work-group per
+  // tile, tile indices in int); the others use gridDim.y for the cube
+  const unsigned max_cubes = p.tiled ? std::max(1u, (unsigned)(0x7fffffff / std::max(1u, p.tiles))) : 65535u;
+  for (unsigned b0 = 0; b0 < nb; b0 += max_cubes) {
+    const unsigned cnt = std::min(max_cubes, nb - b0);
     const void* in = (const char*)p.in + b0 * ce * 8;
     void* out = (char*)p.out + b0 * ce * 8;
     const void* H = p.H ? (const void*)((const char*)p.H + (size_t)b0 * p.hstride * 8) : nullptr;
     long long hs = p.hstride;
     float sc = p.scale;
     const void* twp = tw;
-    void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc};
-    HIPCHK(hipModuleLaunchKernel(p.f, p.tiles, cnt, 1, p.threads, 1, 1, 0, s, args, nullptr));
+    int ntotal = (int)(p.tiles * cnt);
+    void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc, (void*)&ntotal};
+    if (p.tiled) {
+      HIPCHK(hipModuleLaunchKernel(p.f, (unsigned)ntotal, 1, 1, p.threads, 1, 1, 0, s, args, nullptr));
+    } else {
+      HIPCHK(hipModuleLaunchKernel(p.f, p.tiles, cnt, 1, p.threads, 1, 1, 0, s, args, nullptr));
+    }
   }
   return CF_OK;
 }
@@ -229,15 +315,35 @@ void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void*
   v.push_back({f.row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv", io});
 }
 
-// convolution: x fwd, y fwd, z fwd*H*z inv, y inv, x inv(+scale).
-// Returns the index of the first inverse-only pass (for the fpga_t split).
-size_t conv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* sig, void* out, const void* H,
-                   long long hstride, float scale, std::vector<Pass>& v) {
+// Convolution passes of `plan`.  AXIS5: x, y, z*H*z, y^-1, x^-1(+scale), in
+// place in out.  PLANE3: xy-plane, z*H*z, xy-plane^-1(+scale), in place.
+// SPLIT3: sig -> work (pass A), work in place (middle), work -> out (pass C).
+// Returns the index of the first pass after the product (the fpga_t split).
+size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const void* sig, void* out, void* work,
+                   const void* H, long long hstride, float scale, std::vector<Pass>& v) {
   const double V = (double)cube_bytes(n);
   const double io = 2.0 * V * nb;
   const double hbytes = hstride ? V * nb : V;
   const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
   const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+  if (plan == CF_PLAN_PLANE3) {
+    const unsigned pt = cfgeo::plane_threads(n), pT = cfgeo::plane_tiles(n);
+    v.push_back({f.plane_fwd, pT, pt, sig, out, nullptr, 0, 1.0f, "xy_fwd", io});
+    v.push_back({f.colz_conv, cT, ct, out, out, H, hstride, 1.0f, "z_fwd_mul_z_inv", io + hbytes});
+    const size_t split = v.size();
+    v.push_back({f.plane_inv, pT, pt, out, out, nullptr, 0, scale, "xy_inv", io});
+    return split;
+  }
+  if (is_split(plan)) {
+    const int vi = split_variant(plan);
+    const unsigned at = cfgeo::split_a_threads(vi), aT = cfgeo::split_a_tiles(vi);
+    const unsigned bt = cfgeo::split_b_threads(vi), bT = cfgeo::split_b_tiles(vi);
+    v.push_back({f.xy1_fwd[vi], aT, at, sig, work, nullptr, 0, 1.0f, "xy1_fwd", io});
+    v.push_back({f.mid[vi], bT, bt, work, work, H, hstride, 1.0f, "y2z_fwd_mul_inv", io + hbytes});
+    const size_t split = v.size();
+    v.push_back({f.xy1_inv[vi], aT, at, work, out, nullptr, 0, scale, "xy1_inv", io});
+    return split;
+  }
   v.push_back({f.row_fwd, rT, rt, sig, out, nullptr, 0, 1.0f, "x_fwd", io});
   v.push_back({f.coly_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "y_fwd", io});
   v.push_back({f.colz_conv, cT, ct, out, out, H, hstride, 1.0f, "z_fwd_mul_z_inv", io + hbytes});
@@ -245,6 +351,19 @@ size_t conv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* sig, v
   v.push_back({f.coly_inv, cT, ct, out, out, nullptr, 0, 1.0f, "y_inv", io});
   v.push_back({f.row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv", io});
   return split;
+}
+
+// Filter setup of `plan`: natural-order forward FFT3 (in place in dst), plus the
+// re-layout into the middle pass's order for the four-step plans (via work).
+void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, const void* src, void* dst, void* work,
+                     std::vector<Pass>& v) {
+  if (!is_split(plan)) {
+    fwd_passes(f, n, count, src, dst, v);
+    return;
+  }
+  fwd_passes(f, n, count, src, work, v);
+  v.push_back({f.hlay[split_variant(plan)], (unsigned)(cube_elems(n) / 256), 256, work, dst, nullptr, 0, 1.0f,
+               "h_layout", 2.0 * cube_bytes(n) * count, false});
 }
 
 int run_passes(const std::vector<Pass>& v, unsigned n, unsigned nb, hipStream_t s, const void* tw,
@@ -279,6 +398,34 @@ int ensure_pinned(void** p, size_t* have, size_t need) {
   return CF_OK;
 }
 
+// Workspace of `bytes` for a call on stream s.  Calls on different streams are
+// ordered through work_ev so that they never overlap on the shared workspace.
+int acquire_work(size_t bytes, hipStream_t s, void** out) {
+  if (!g.work_ev) HIPCHK(hipEventCreateWithFlags(&g.work_ev, hipEventDisableTiming));
+  if (g.d_work_bytes < bytes) {
+    if (g.d_work) {
+      HIPCHK(hipDeviceSynchronize());  // earlier users on any stream are done
+      hipFree(g.d_work);
+    }
+    g.d_work = nullptr;
+    g.d_work_bytes = 0;
+    g.work_used = false;
+    hipError_t e = hipMalloc(&g.d_work, bytes);
+    if (e != hipSuccess) return fail(CF_ERR_NOMEM, "hipMalloc(%zu) for the workspace: %s", bytes, hipGetErrorString(e));
+    g.d_work_bytes = bytes;
+  }
+  if (g.work_used && g.work_stream != s) HIPCHK(hipStreamWaitEvent(s, g.work_ev, 0));
+  *out = g.d_work;
+  return CF_OK;
+}
+
+int release_work(hipStream_t s) {
+  HIPCHK(hipEventRecord(g.work_ev, s));
+  g.work_stream = s;
+  g.work_used = true;
+  return CF_OK;
+}
+
 int ensure_events() {
   for (auto& e : g.ev)
     if (!e) HIPCHK(hipEventCreate(&e));
@@ -295,6 +442,77 @@ void* mapped(void* host) {
   void* d = nullptr;
   if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return host;
   return d;
+}
+
+// Times every supported plan on one scratch cube (2 warm-up and 5 timed
+// convolutions each, HIP events on the library stream) and keeps the fastest.
+int autotune(unsigned n, SizeFns* f) {
+  std::vector<int> cand;
+  for (int p = 1; p < CF_NUM_PLANS; ++p)
+    if (plan_ok(n, p)) cand.push_back(p);
+  for (auto& m : f->tune_ms) m = 0.0f;
+  if (cand.size() == 1) {
+    f->tuned = cand[0];
+    return CF_OK;
+  }
+  const size_t V = cube_bytes(n);
+  void *sig = nullptr, *out = nullptr, *H = nullptr;
+  void* work = nullptr;
+  hipStream_t s = g.stream;
+  int rc = CF_OK;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(s);
+    if (sig) hipFree(sig);
+    if (out) hipFree(out);
+    if (H) hipFree(H);
+  };
+  if (hipMalloc(&sig, V) != hipSuccess || hipMalloc(&out, V) != hipSuccess || hipMalloc(&H, V) != hipSuccess) {
+    cleanup();
+    return fail(CF_ERR_NOMEM, "autotune scratch allocation failed");
+  }
+  rc = acquire_work(V, s, &work);
+  long long nf = (long long)(V / 4);
+  unsigned long long seed = 12345;
+  void* fa[] = {&sig, &nf, &seed};
+  void* fb[] = {&H, &nf, &seed};
+  if (rc == CF_OK && (hipModuleLaunchKernel(g.fillu, 1024, 1, 1, 256, 1, 1, 0, s, fa, nullptr) != hipSuccess ||
+                      hipModuleLaunchKernel(g.fillu, 1024, 1, 1, 256, 1, 1, 0, s, fb, nullptr) != hipSuccess))
+    rc = fail(CF_ERR_RUNTIME, "autotune fill failed");
+  float best = 0.0f;
+  int best_plan = cand[0];
+  for (int p : cand) {
+    if (rc != CF_OK) break;
+    std::vector<Pass> v;
+    conv_passes(*f, p, n, 1, sig, out, work, H, 0, 1.0f, v);
+    for (int w = 0; w < 2 && rc == CF_OK; ++w) rc = run_passes(v, n, 1, s, f->tw, nullptr);
+    if (rc == CF_OK && hipEventRecord(g.ev[0], s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "event");
+    for (int r = 0; r < 5 && rc == CF_OK; ++r) rc = run_passes(v, n, 1, s, f->tw, nullptr);
+    if (rc == CF_OK && (hipEventRecord(g.ev[1], s) != hipSuccess || hipEventSynchronize(g.ev[1]) != hipSuccess))
+      rc = fail(CF_ERR_RUNTIME, "event");
+    if (rc != CF_OK) break;
+    const float ms = (float)(ms_between(g.ev[0], g.ev[1]) / 5.0);
+    f->tune_ms[p] = ms;
+    if (best == 0.0f || ms < best) {
+      best = ms;
+      best_plan = p;
+    }
+  }
+  if (rc == CF_OK) rc = release_work(s);
+  cleanup();
+  if (rc != CF_OK) return rc;
+  f->tuned = best_plan;
+  write_tune_file(n, *f);
+  return CF_OK;
+}
+
+int choose_plan(unsigned n, SizeFns* f, int* plan) {
+  if (f->forced != CF_PLAN_AUTO) {
+    *plan = f->forced;
+    return CF_OK;
+  }
+  if (f->tuned == CF_PLAN_AUTO) CFCHK(autotune(n, f));
+  *plan = f->tuned;
+  return CF_OK;
 }
 
 }  // namespace
@@ -316,6 +534,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
   if (ndev > 0 && hipGetDeviceProperties(&prop, 0) == hipSuccess) {
     names.push_back(prop.name);
     names.push_back(prop.gcnArchName);
+    g.arch = std::string(prop.gcnArchName).substr(0, std::string(prop.gcnArchName).find(':'));
   }
   const std::string want = lower(platform_name);
   bool match = false;
@@ -360,70 +579,78 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
                              cfe_times_t* t) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!sig || !filter || !out || how_many == 0 || !t) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  if (mode != CFE_MODE_COPY && mode != CFE_MODE_SVM) return fail(CF_ERR_ARG, "unknown mode %d", mode);
   CFCHK(require_ready());
+  if (mode == CFE_MODE_SVM && !g.svm) return fail(CF_ERR_STATE, "fpga_initialize() was called with use_svm = false");
   SizeFns* f = nullptr;
   CFCHK(size_res(n, &f));
+  int plan = 0;
+  CFCHK(choose_plan(n, f, &plan));
   const size_t V = cube_bytes(n);
   const size_t SV = V * how_many;
   memset(t, 0, sizeof *t);
   hipEvent_t* ev = g.ev;
   hipStream_t s = g.stream;
+  CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
+  CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
+  void* work = nullptr;
+  if (is_split(plan)) CFCHK(acquire_work(SV, s, &work));
   std::vector<Pass> filt_p, conv_p;
 
   if (mode == CFE_MODE_COPY) {
-    CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
-    CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
     // step 1-2: filter H2D and transform (convfpga.c:75-172)
     HIPCHK(hipEventRecord(ev[0], s));
     HIPCHK(hipMemcpyAsync(g.d_filt, filter, V, hipMemcpyHostToDevice, s));
-    fwd_passes(*f, n, 1, g.d_filt, g.d_filt, filt_p);
-    CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev + 1));  // ev[1..4]
+    spectrum_passes(*f, plan, n, 1, g.d_filt, g.d_filt, work, filt_p);
+    const size_t nf = filt_p.size();
+    CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev + 1));  // ev[1 .. 1+nf]
     // step 3: signal H2D, forward + product (convfpga.c:174-279)
     HIPCHK(hipMemcpyAsync(g.d_data, sig, SV, hipMemcpyHostToDevice, s));
-    const size_t split = conv_passes(*f, n, how_many, g.d_data, g.d_data, g.d_filt, 0, 1.0f, conv_p);
-    CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ev + 5));  // ev[5..10]
-    // step 4 done by the last passes; D2H (convfpga.c:365-375)
+    const size_t split = conv_passes(*f, plan, n, how_many, g.d_data, g.d_data, work, g.d_filt, 0, 1.0f, conv_p);
+    const size_t nc = conv_p.size();
+    hipEvent_t* ec = ev + 2 + nf;
+    CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ec));  // ec[0 .. nc]
+    if (work) CFCHK(release_work(s));
+    // step 4 ran in the last passes; D2H (convfpga.c:365-375)
     HIPCHK(hipMemcpyAsync(out, g.d_data, SV, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(ev[11], s));
+    HIPCHK(hipEventRecord(ec[nc + 1], s));
     HIPCHK(hipStreamSynchronize(s));
     t->filter_pcie_wr_t = ms_between(ev[0], ev[1]);
-    t->filter_exec_t = ms_between(ev[1], ev[4]);
-    t->sig_pcie_wr_t = ms_between(ev[4], ev[5]);
-    t->sig_exec_t = ms_between(ev[5], ev[5 + split]);
-    t->siginv_exec_t = ms_between(ev[5 + split], ev[10]);
-    t->sig_pcie_rd_t = ms_between(ev[10], ev[11]);
+    t->filter_exec_t = ms_between(ev[1], ev[1 + nf]);
+    t->sig_pcie_wr_t = ms_between(ev[1 + nf], ec[0]);
+    t->sig_exec_t = ms_between(ec[0], ec[split]);
+    t->siginv_exec_t = ms_between(ec[split], ec[nc]);
+    t->sig_pcie_rd_t = ms_between(ec[nc], ec[nc + 1]);
     return CF_OK;
   }
-  if (mode == CFE_MODE_SVM) {
-    if (!g.svm) return fail(CF_ERR_STATE, "fpga_initialize() was called with use_svm = false");
-    CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
-    CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
-    CFCHK(ensure_pinned(&g.h_filt, &g.h_filt_bytes, V));
-    CFCHK(ensure_pinned(&g.h_sig, &g.h_sig_bytes, SV));
-    CFCHK(ensure_pinned(&g.h_out, &g.h_out_bytes, SV));
-    // host copies into the mapped buffers (the reference's SVM map + memcpy,
-    // convfpga_svm.c:86-103); the kernels then read them over the bus.
-    memcpy(g.h_filt, filter, V);
-    memcpy(g.h_sig, sig, SV);
-    fwd_passes(*f, n, 1, mapped(g.h_filt), g.d_filt, filt_p);
-    CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev));  // ev[0..3]
-    const size_t split = conv_passes(*f, n, how_many, mapped(g.h_sig), g.d_data, g.d_filt, 0, 1.0f, conv_p);
-    conv_p.back().out = mapped(g.h_out);  // last pass writes host memory directly
-    CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ev + 4));  // ev[4..9]
-    HIPCHK(hipStreamSynchronize(s));
-    memcpy(out, g.h_out, SV);
-    t->filter_exec_t = ms_between(ev[0], ev[3]);
-    if (how_many == 1) {
-      t->sig_exec_t = ms_between(ev[4], ev[4 + split]);
-      t->siginv_exec_t = ms_between(ev[4 + split], ev[9]);
-    } else {
-      // batch: summed stage time and first-to-last wall span (convfpga_svm_batch.c:664-665)
-      t->sig_exec_t = ms_between(ev[4], ev[4 + split]) + ms_between(ev[4 + split], ev[9]);
-      t->siginv_exec_t = ms_between(ev[4], ev[9]);
-    }
-    return CF_OK;
+  CFCHK(ensure_pinned(&g.h_filt, &g.h_filt_bytes, V));
+  CFCHK(ensure_pinned(&g.h_sig, &g.h_sig_bytes, SV));
+  CFCHK(ensure_pinned(&g.h_out, &g.h_out_bytes, SV));
+  // host copies into the mapped buffers (the reference's SVM map + memcpy,
+  // convfpga_svm.c:86-103); the first and last passes then use them over the bus.
+  memcpy(g.h_filt, filter, V);
+  memcpy(g.h_sig, sig, SV);
+  spectrum_passes(*f, plan, n, 1, mapped(g.h_filt), g.d_filt, work, filt_p);
+  const size_t nf = filt_p.size();
+  CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev));  // ev[0 .. nf]
+  const size_t split = conv_passes(*f, plan, n, how_many, mapped(g.h_sig), g.d_data, work, g.d_filt, 0, 1.0f, conv_p);
+  conv_p.back().out = mapped(g.h_out);  // the last pass writes host memory directly
+  const size_t nc = conv_p.size();
+  hipEvent_t* ec = ev + nf + 1;
+  CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ec));  // ec[0 .. nc]
+  if (work) CFCHK(release_work(s));
+  HIPCHK(hipStreamSynchronize(s));
+  memcpy(out, g.h_out, SV);
+  t->filter_exec_t = ms_between(ev[0], ev[nf]);
+  if (how_many == 1) {
+    t->sig_exec_t = ms_between(ec[0], ec[split]);
+    t->siginv_exec_t = ms_between(ec[split], ec[nc]);
+  } else {
+    // batch: summed stage time and first-to-last wall span (convfpga_svm_batch.c:664-665)
+    t->sig_exec_t = ms_between(ec[0], ec[split]) + ms_between(ec[split], ec[nc]);
+    t->siginv_exec_t = ms_between(ec[0], ec[nc]);
   }
-  return fail(CF_ERR_ARG, "unknown mode %d", mode);
+  return CF_OK;
 }
 
 // =============================================================================
@@ -433,6 +660,7 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
 #pragma GCC visibility push(default)
 struct cf_spectrum_s {
   unsigned n = 0, count = 0;
+  int plan = CF_PLAN_AXIS5;
   void* d = nullptr;
 };
 
@@ -463,10 +691,16 @@ extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filter
     e = hipMemcpyAsync(h->d, filters, bytes, hipMemcpyHostToDevice, s);
     src = h->d;
   }
-  std::vector<Pass> p;
-  fwd_passes(*f, n, count, src, h->d, p);
-  int rc = e == hipSuccess ? run_passes(p, n, count, s, f->tw, nullptr)
+  int rc = e == hipSuccess ? choose_plan(n, f, &h->plan)
                            : fail(CF_ERR_RUNTIME, "hipMemcpyAsync: %s", hipGetErrorString(e));
+  void* work = nullptr;
+  if (rc == CF_OK && is_split(h->plan)) rc = acquire_work(bytes, s, &work);
+  if (rc == CF_OK) {
+    std::vector<Pass> p;
+    spectrum_passes(*f, h->plan, n, count, src, h->d, work, p);
+    rc = run_passes(p, n, count, s, f->tw, nullptr);
+    if (rc == CF_OK && work) rc = release_work(s);
+  }
   if (rc == CF_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "spectrum setup failed");
   if (rc != CF_OK) {
     hipFree(h->d);
@@ -488,6 +722,40 @@ extern "C" void cf_spectrum_destroy(cf_spectrum_t h) {
 }
 
 extern "C" const void* cf_spectrum_data(cf_spectrum_t h) { return h ? h->d : nullptr; }
+extern "C" int cf_spectrum_plan(cf_spectrum_t h) { return h ? h->plan : CF_ERR_ARG; }
+
+extern "C" int cf_plan_supported(unsigned n, int plan) { return plan_ok(n, plan) ? 1 : 0; }
+
+extern "C" int cf_set_plan(unsigned n, int plan) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (plan != CF_PLAN_AUTO && !plan_ok(n, plan)) return fail(CF_ERR_ARG, "plan %d cannot run N=%u", plan, n);
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  f->forced = plan;
+  return CF_OK;
+}
+
+extern "C" int cf_get_plan(unsigned n) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  int plan = 0;
+  CFCHK(choose_plan(n, f, &plan));
+  return plan;
+}
+
+extern "C" int cf_autotune(unsigned n, float* ms) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(n, &f));
+  CFCHK(autotune(n, f));
+  if (ms)
+    for (int p = 0; p < CF_NUM_PLANS; ++p) ms[p] = f->tune_ms[p];
+  return f->tuned;
+}
 extern "C" unsigned cf_spectrum_edge(cf_spectrum_t h) { return h ? h->n : 0; }
 extern "C" unsigned cf_spectrum_count(cf_spectrum_t h) { return h ? h->count : 0; }
 
@@ -504,9 +772,18 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
   const unsigned n = h->n;
   const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
   const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
-  std::vector<Pass> p;
-  conv_passes(*f, n, how_many, d_sig, d_out, h->d, hstride, scale, p);
   hipStream_t s = (hipStream_t)stream;
+  void* work = nullptr;
+  if (is_split(h->plan)) CFCHK(acquire_work(cube_bytes(n) * how_many, s, &work));
+  std::vector<Pass> p;
+  conv_passes(*f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p);
+  struct WorkRelease {  // the workspace is marked busy on s whatever path returns
+    void* w;
+    hipStream_t s;
+    ~WorkRelease() {
+      if (w) release_work(s);
+    }
+  } wr{work, s};
   if (!times) {
     if (g.prof_max > g.prof_calls) {  // armed by cf_profile_begin
       hipEvent_t* ev = &g.prof_ev[(size_t)g.prof_calls * (CF_MAX_PASSES + 1)];

@@ -27,4 +27,17 @@ constexpr int row_tiles(int n) { return n * n / row_l(n); }
 
 constexpr bool supported(int n) { return points_per_thread(n) != 0; }
 
+// 3-pass plane schedule (N <= 128): one xy-plane per work-group
+constexpr bool plane_supported(int n) { return n == 32 || n == 64 || n == 128; }
+constexpr int plane_threads(int n) { return n * threads_per_line(n); }
+constexpr int plane_tiles(int n) { return n; }
+
+// 3-pass four-step schedule (N = 256): tile groups GA (pass A/C) and GB (middle)
+constexpr int split_ga(int variant) { return variant == 0 ? 2 : variant == 1 ? 2 : variant == 2 ? 4 : 4; }
+constexpr int split_gb(int variant) { return variant == 0 ? 2 : variant == 1 ? 4 : variant == 2 ? 2 : 4; }
+constexpr int split_a_threads(int variant) { return 256 * split_ga(variant); }
+constexpr int split_a_tiles(int variant) { return 256 * (16 / split_ga(variant)); }
+constexpr int split_b_threads(int variant) { return 256 * split_gb(variant); }
+constexpr int split_b_tiles(int variant) { return 16 * (256 / split_gb(variant)); }
+
 }  // namespace cfgeo

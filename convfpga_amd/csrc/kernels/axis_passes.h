@@ -1,0 +1,142 @@
+// axis_passes.h -- one-axis passes of the 5-pass schedule (and of the plain
+// 3D FFT), as tile kernels (one tile per work-group, run_tiles of fft_core.h).
+//
+// What replaces what in the reference pipeline (kernels/conv3D.cl):
+//   RowPass   : fetch + fft3da (24-116): N-point FFT along contiguous x.
+//   ColPass y : transpose + fft3db (118-232): FFT along y (stride N), 16
+//               adjacent x-columns per tile so every load/store instruction
+//               touches 128 contiguous bytes per 16 lanes.
+//   ColPass z : transpose3D + fft3dc + store (234-515): FFT along z (stride N^2).
+//   ColPass z, MODE 2: fft3dc forward, the conv3D product with the filter
+//               spectrum (517-570) and the inverse fft3dc in one pass: the
+//               spectrum of a z-line never leaves registers.
+#pragma once
+#include "fft_core.h"
+
+namespace cfk {
+
+template <int N>
+struct ColCfg {
+  static constexpr int P = Shape<N>::P, T = N / P;
+  static constexpr int C = N < 16 ? N : 16;  // adjacent x-columns per tile
+  static constexpr int G0 = 256 / (C * T);
+  static constexpr int G = G0 < 1 ? 1 : (G0 > N ? N : G0);  // lines of the third axis per tile
+  static constexpr int THREADS = C * T * G;
+  static constexpr int TILES = (N / C) * (N / G);  // tiles per cube
+  static constexpr int LDS = ShapeInfo<N>::NSTEPS > 1 ? G * N * (C + 1) : 1;
+};
+
+template <int N>
+struct RowCfg {
+  static constexpr int P = Shape<N>::P, T = N / P;
+  static constexpr int L0 = 256 / T;
+  static constexpr int L = L0 > N * N ? N * N : L0;  // rows per tile
+  static constexpr int THREADS = L * T;
+  static constexpr int TILES = N * N / L;
+  static constexpr int PITCH = N + N / 16;
+  static constexpr int LDS = ShapeInfo<N>::NSTEPS > 1 ? L * PITCH : 1;
+};
+
+// x-axis pass; SCALE multiplies the result by `scale` (last inverse pass).
+template <int N, bool INV, bool SCALE>
+struct RowPass {
+  using K = RowCfg<N>;
+  static constexpr int P = K::P, T = K::T;
+  struct Regs {
+    float2 v[P];
+  };
+  const float2* in;
+  float2* out;
+  const float2* tw_s;
+  float2* lds;
+  float scale;
+  int t, l;
+
+  __device__ __forceinline__ size_t base(int tile) const {
+    const int cube = tile / K::TILES, tt = tile % K::TILES;
+    return (size_t)cube * N * N * N + (size_t)(tt * K::L + l) * N + t;
+  }
+  __device__ __forceinline__ void load(int tile, Regs& r) const {
+    const float2* src = in + base(tile);
+#pragma unroll
+    for (int m = 0; m < P; ++m) r.v[m] = src[T * m];
+  }
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    fft_line<N, INV>(r.v, t, RowLds{lds + l * K::PITCH}, tw_s);
+    float2* dst = out + base(tile);
+#pragma unroll
+    for (int m = 0; m < P; ++m) dst[T * m] = SCALE ? cscale(r.v[m], scale) : r.v[m];
+  }
+};
+
+// y (AXIS 1) or z (AXIS 2) pass.  MODE 0 forward, 1 inverse, 2 forward * H
+// then inverse (z only).
+template <int N, int AXIS, int MODE>
+struct ColPass {
+  using K = ColCfg<N>;
+  static constexpr int P = K::P, T = K::T, C = K::C, G = K::G;
+  static constexpr int S = AXIS == 1 ? N : N * N;  // element stride along the axis
+  struct Regs {
+    float2 v[P];
+  };
+  const float2* in;
+  float2* out;
+  const float2* H;
+  const float2* tw_s;
+  float2* lds;
+  long long hstride;
+  int c, t, g;
+
+  __device__ __forceinline__ int offset(int tt) const {
+    const int xg = tt % (N / C);
+    const int other = (tt / (N / C)) * G + g;  // z for the y pass, y for the z pass
+    return xg * C + c + (AXIS == 1 ? other * N * N : other * N) + t * S;
+  }
+  __device__ __forceinline__ void load(int tile, Regs& r) const {
+    const int cube = tile / K::TILES, tt = tile % K::TILES;
+    const float2* src = in + (size_t)cube * N * N * N + offset(tt);
+#pragma unroll
+    for (int m = 0; m < P; ++m) r.v[m] = src[T * S * m];
+  }
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    const int cube = tile / K::TILES, tt = tile % K::TILES;
+    const int off = offset(tt);
+    const ColLds<C> view{lds + g * N * (C + 1) + c};
+    if constexpr (MODE == 2) {
+      fft_line<N, false>(r.v, t, view, tw_s);
+      const float2* hp = H + (size_t)cube * hstride + off;
+#pragma unroll
+      for (int m = 0; m < P; ++m) r.v[m] = cmul(r.v[m], hp[T * S * m]);
+      fft_line<N, true>(r.v, t, view, tw_s);
+    } else {
+      fft_line<N, MODE == 1>(r.v, t, view, tw_s);
+    }
+    float2* dst = out + (size_t)cube * N * N * N + off;
+#pragma unroll
+    for (int m = 0; m < P; ++m) dst[T * S * m] = r.v[m];
+  }
+};
+
+template <int N, bool INV, bool SCALE>
+__device__ __forceinline__ void row_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
+  using K = RowCfg<N>;
+  __shared__ float2 lds[K::LDS];
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, K::THREADS>(tw_s, tw);
+  const RowPass<N, INV, SCALE> k{in, out, tw_s, lds, scale, (int)threadIdx.x % K::T, (int)threadIdx.x / K::T};
+  run_tiles(k, ntotal);
+}
+
+template <int N, int AXIS, int MODE>
+__device__ __forceinline__ void col_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
+                                           long long hstride, int ntotal) {
+  using K = ColCfg<N>;
+  __shared__ float2 lds[K::LDS];
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, K::THREADS>(tw_s, tw);
+  const int c = threadIdx.x % K::C, t = (threadIdx.x / K::C) % K::T, g = threadIdx.x / (K::C * K::T);
+  const ColPass<N, AXIS, MODE> k{in, out, H, tw_s, lds, hstride, c, t, g};
+  run_tiles(k, ntotal);
+}
+
+}  // namespace cfk

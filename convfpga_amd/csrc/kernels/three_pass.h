@@ -1,0 +1,311 @@
+// three_pass.h -- 3-pass convolution kernels: 7 volume reads/writes per
+// convolution instead of the 11 of the axis-by-axis schedule.
+//
+// N <= 128 (an N x N plane fits one work-group's LDS):
+//   plane_fwd : 2D FFT of one xy-plane (x then y), natural order in and out
+//   colz_conv : z forward, product with the spectrum, z inverse (axis pass kernel)
+//   plane_inv : inverse 2D FFT of one xy-plane (y then x) with the final scale
+//
+// N = 256 (a plane does not fit): the y transform is split four-step style,
+// y = 16*y1 + y2, ky = k1 + 16*k2:
+//   X[k1 + 16 k2] = sum_y2 W16^(y2 k2) W256^(y2 k1) [ sum_y1 x[16 y1 + y2] W16^(y1 k1) ]
+//   xy1_fwd : x FFT of 16*GA rows, 16-point DFT over y1, twiddle W256^(y2 k1)
+//   mid_conv: z FFT, 16-point DFT over y2, product with the spectrum, inverse
+//             16-point DFT over k2, inverse z FFT -- one contiguous tile
+//   xy1_inv : twiddle W256^(-y2 k1), inverse 16-point DFT over k1, inverse x
+//             FFT, final scale, natural order out
+// The intermediate between the passes (in the workspace) is indexed by
+// (x, k1, y2, z) and laid out [k1][x/GB][z][y2][x%GB], so that a mid_conv tile
+// (k1, x/GB) is one contiguous block of 4096*GB points and xy1_* move runs of
+// GA*GB contiguous points.  The filter spectrum for mid_conv is stored
+// [k1][x/GB][k2][kz][x%GB] so that it is read with unit stride as well.
+#pragma once
+#include "fft_core.h"
+
+namespace cfk {
+
+// ---------------------------------------------------------------------------
+// N <= 128: whole xy-plane per work-group
+// ---------------------------------------------------------------------------
+template <int N>
+struct PlaneCfg {
+  static constexpr int P = Shape<N>::P, T = N / P;
+  static constexpr int THREADS = N * T;     // T threads per row / per column
+  static constexpr int ROW_PITCH = N + N / 16;
+  static constexpr int COL_PITCH = N + 1;
+  static constexpr int LDS = N * (ROW_PITCH > COL_PITCH ? ROW_PITCH : COL_PITCH);
+};
+
+// Column view of the plane buffer: (point y, column kx) at y*(N+1) + kx.
+template <int N>
+struct PlaneColLds {
+  float2* p;  // offset by the column
+  __device__ __forceinline__ void st(int o, float2 x) const { p[o * (N + 1)] = x; }
+  __device__ __forceinline__ float2 ld(int o) const { return p[o * (N + 1)]; }
+};
+
+// INV = false: rows (x) then columns (y), natural in -> natural out.
+// INV = true : columns then rows, and the result is multiplied by `scale`.
+template <int N, bool INV>
+struct PlanePass {
+  using K = PlaneCfg<N>;
+  static constexpr int P = K::P, T = K::T;
+  struct Regs {
+    float2 v[P];
+  };
+  const float2* in;
+  float2* out;
+  const float2* tw_s;
+  float2* lds;
+  float scale;
+  int rt, ry;  // row identity: row y, thread t of the row (points x = t + T*m)
+  int ck, ct;  // column identity: column kx (fastest across lanes), thread t of the column
+
+  __device__ __forceinline__ void load(int tile, Regs& r) const {
+    const float2* src = in + (size_t)tile * N * N;
+    if constexpr (!INV) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) r.v[m] = src[ry * N + rt + T * m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) r.v[m] = src[(ct + T * m) * N + ck];
+    }
+  }
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    float2* dst = out + (size_t)tile * N * N;
+    if constexpr (!INV) {
+      fft_line<N, false>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);
+#pragma unroll
+      for (int m = 0; m < P; ++m) lds[ry * (N + 1) + rt + T * m] = r.v[m];  // rows -> columns
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < P; ++m) r.v[m] = lds[(ct + T * m) * (N + 1) + ck];
+      __syncthreads();
+      fft_line<N, false>(r.v, ct, PlaneColLds<N>{lds + ck}, tw_s);
+#pragma unroll
+      for (int m = 0; m < P; ++m) dst[(ct + T * m) * N + ck] = r.v[m];
+    } else {
+      fft_line<N, true>(r.v, ct, PlaneColLds<N>{lds + ck}, tw_s);
+#pragma unroll
+      for (int m = 0; m < P; ++m) lds[(ct + T * m) * (N + 1) + ck] = r.v[m];  // columns -> rows
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < P; ++m) r.v[m] = lds[ry * (N + 1) + rt + T * m];
+      __syncthreads();
+      fft_line<N, true>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);
+#pragma unroll
+      for (int m = 0; m < P; ++m) dst[ry * N + rt + T * m] = cscale(r.v[m], scale);
+    }
+  }
+};
+
+template <int N, bool INV>
+__device__ __forceinline__ void plane_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
+  using K = PlaneCfg<N>;
+  __shared__ float2 lds[K::LDS];
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, K::THREADS>(tw_s, tw);
+  const int x = threadIdx.x;
+  const PlanePass<N, INV> k{in, out, tw_s, lds, scale, x % K::T, x / K::T, x % N, x / N};
+  run_tiles(k, ntotal);  // tile = plane index over the whole batch
+}
+
+// ---------------------------------------------------------------------------
+// N = 256: four-step split of y
+// ---------------------------------------------------------------------------
+template <int GA, int GB>
+struct Split256 {
+  static constexpr int N = 256;
+  static constexpr int NXH = N / GB;               // x blocks
+  static constexpr int A_THREADS = 256 * GA;       // 16*GA rows of 16 threads (launch bound of xy1_*)
+  static constexpr int A_TILES = N * (16 / GA);    // (z, y2 group) per cube
+  static constexpr int A_ROW_PITCH = 272;          // 256 + 256/16
+  static constexpr int A_COL_PLANE = 256 * 17 + 4; // (kx, y1) block of one y2, padded
+  static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
+  static constexpr int B_THREADS = 256 * GB;       // 16*GB z-lines of 16 threads (launch bound of mid)
+  static constexpr int B_TILES = 16 * NXH;         // (k1, x block) per cube
+  static constexpr int B_LINES = 16 * GB;          // z-lines per tile: (y2, x%GB)
+  static constexpr int B_LDS = 256 * (B_LINES + 1);
+  static constexpr int TILE = 4096 * GB;           // points per mid tile
+  // intermediate index of (x, k1, y2, z)
+  __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
+    return (((k1 * NXH + x / GB) * 256 + z) * 16 + y2) * GB + (x % GB);
+  }
+};
+
+// INV = false: pass A (natural -> intermediate).  INV = true: pass C.
+template <int GA, int GB, bool INV>
+struct Xy1Pass {
+  using K = Split256<GA, GB>;
+  static constexpr int N = 256;
+  static constexpr int K1_STRIDE = K::NXH * 256 * 16 * GB;  // mid_index step of k1
+  struct Regs {
+    float2 v[16];
+  };
+  const float2* in;
+  float2* out;
+  const float2* tw_s;
+  float2* lds;
+  float scale;
+  int rt, row, ry1, ry2l;  // row identity: row = y2l*16 + y1, thread t holds x = t + 16*m
+  int ckx, cy2l;           // column identity: (x, y2l) with x%GB fastest, then y2l, then x/GB
+
+  __device__ __forceinline__ void tile_pos(int tile, size_t& cube, int& z, int& y2_0) const {
+    cube = (size_t)(tile / K::A_TILES) * N * N * N;
+    const int tt = tile % K::A_TILES;
+    z = tt / (16 / GA);
+    y2_0 = (tt % (16 / GA)) * GA;
+  }
+  __device__ __forceinline__ void load(int tile, Regs& r) const {
+    size_t cube;
+    int z, y2_0;
+    tile_pos(tile, cube, z, y2_0);
+    if constexpr (!INV) {
+      const float2* src = in + cube + (size_t)z * N * N + (16 * ry1 + y2_0 + ry2l) * N + rt;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) r.v[m] = src[16 * m];
+    } else {
+      const float2* src = in + cube + K::mid_index(ckx, 0, y2_0 + cy2l, z);
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) r.v[k1] = src[k1 * K1_STRIDE];
+    }
+  }
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    size_t cube;
+    int z, y2_0;
+    tile_pos(tile, cube, z, y2_0);
+    const int cy2 = y2_0 + cy2l;
+    float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;  // this thread's 16 y1 (or k1) slots
+    if constexpr (!INV) {
+      fft_line<N, false>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + 16 * m) * 17 + ry1] = r.v[m];
+      __syncthreads();
+      float2 u[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[j] = colp[j];
+      __syncthreads();  // LDS free for the next tile
+      DFT<16, false>::run(u);  // u[k1]
+#pragma unroll
+      for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulw<false>(u[k1], tw_s[cy2 * k1]);
+      float2* dst = out + cube + K::mid_index(ckx, 0, cy2, z);
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) dst[k1 * K1_STRIDE] = u[k1];
+    } else {
+#pragma unroll
+      for (int k1 = 1; k1 < 16; ++k1) r.v[k1] = cmulw<true>(r.v[k1], tw_s[cy2 * k1]);
+      DFT<16, true>::run(r.v);  // r.v[y1]
+#pragma unroll
+      for (int j = 0; j < 16; ++j) colp[j] = r.v[j];
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + 16 * m) * 17 + ry1];
+      __syncthreads();
+      fft_line<N, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
+      float2* dst = out + cube + (size_t)z * N * N + (16 * ry1 + y2_0 + ry2l) * N + rt;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) dst[16 * m] = cscale(r.v[m], scale);
+    }
+  }
+};
+
+template <int GA, int GB, bool INV>
+__device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
+  using K = Split256<GA, GB>;
+  __shared__ float2 lds[K::A_LDS];
+  __shared__ float2 tw_s[256];
+  stage_twiddles<256, K::A_THREADS>(tw_s, tw);
+  const int x = threadIdx.x;
+  const int row = x / 16;
+  const int cxl = x % GB, cy2l = (x / GB) % GA, cxh = x / (GB * GA);
+  const Xy1Pass<GA, GB, INV> k{in, out, tw_s, lds, scale, x % 16, row, row % 16, row / 16, cxh * GB + cxl, cy2l};
+  run_tiles(k, ntotal);
+}
+
+// Middle pass: one contiguous tile (k1, x block) = [z][y2][x%GB].
+template <int GA, int GB>
+struct MidPass {
+  using K = Split256<GA, GB>;
+  static constexpr int N = 256, L = K::B_LINES;
+  struct Regs {
+    float2 v[16];
+  };
+  const float2* in;
+  float2* out;
+  const float2* H;
+  const float2* tw_s;
+  float2* lds;
+  long long hstride;
+  int l, zt;  // z-line identity: line l = y2*GB + x%GB, thread zt holds z = zt + 16*m
+  int qx, qz; // (kz, x%GB) identity for the y2 transform
+
+  __device__ __forceinline__ size_t tile_off(int tile) const {
+    return (size_t)(tile / K::B_TILES) * N * N * N + (size_t)(tile % K::B_TILES) * K::TILE;
+  }
+  __device__ __forceinline__ void load(int tile, Regs& r) const {
+    const float2* src = in + tile_off(tile) + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) r.v[m] = src[K::B_THREADS * m];
+  }
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    const ColLds<L> view{lds + l};
+    fft_line<N, false>(r.v, zt, view, tw_s);  // r.v[m] = Z(kz = zt + 16 m) of line l
+#pragma unroll
+    for (int m = 0; m < 16; ++m) lds[(zt + 16 * m) * (L + 1) + l] = r.v[m];
+    __syncthreads();
+    float2 u[16];
+    float2* rowp = lds + qz * (L + 1) + qx;
+#pragma unroll
+    for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GB];
+    const int cube = tile / K::B_TILES;
+    const float2* h = H + (size_t)cube * hstride + (size_t)(tile % K::B_TILES) * K::TILE + qz * GB + qx;
+    float2 hv[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GB];
+    DFT<16, false>::run(u);  // u[k2]: full 3D spectrum at (x, k1 + 16 k2, qz)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) u[k2] = cmul(u[k2], hv[k2]);
+    DFT<16, true>::run(u);  // u[y2]
+    __syncthreads();  // every thread has read its row before it is overwritten
+#pragma unroll
+    for (int y2 = 0; y2 < 16; ++y2) rowp[y2 * GB] = u[y2];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + 16 * m) * (L + 1) + l];
+    __syncthreads();
+    fft_line<N, true>(r.v, zt, view, tw_s);
+    float2* dst = out + tile_off(tile) + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) dst[K::B_THREADS * m] = r.v[m];
+  }
+};
+
+template <int GA, int GB>
+__device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
+                                           long long hstride, int ntotal) {
+  using K = Split256<GA, GB>;
+  __shared__ float2 lds[K::B_LDS];
+  __shared__ float2 tw_s[256];
+  stage_twiddles<256, K::B_THREADS>(tw_s, tw);
+  const int x = threadIdx.x;
+  const MidPass<GA, GB> k{in, out, H, tw_s, lds, hstride, x % K::B_LINES, x / K::B_LINES, x % GB, x / GB};
+  run_tiles(k, ntotal);
+}
+
+// Spectrum re-layout for mid_body: natural FFT3 order -> [k1][x/GB][k2][kz][x%GB].
+template <int GA, int GB>
+__device__ __forceinline__ void h_layout_body(const float2* __restrict__ in, float2* __restrict__ out) {
+  using K = Split256<GA, GB>;
+  constexpr int N = 256;
+  const int i = blockIdx.x * 256 + threadIdx.x;  // output index within the cube
+  const size_t cube = (size_t)blockIdx.y * N * N * N;
+  const int xl = i % GB;
+  const int kz = (i / GB) % 256;
+  const int k2 = (i / (GB * 256)) % 16;
+  const int xh = (i / (GB * 256 * 16)) % K::NXH;
+  const int k1 = i / (GB * 256 * 16 * K::NXH);
+  const int x = xh * GB + xl, ky = k1 + 16 * k2;
+  out[cube + i] = in[cube + ((size_t)kz * N + ky) * N + x];
+}
+
+}  // namespace cfk

@@ -33,9 +33,33 @@ extern "C" {
 #define CF_PREC_FP32 0        /* FP32 FMA butterflies                        */
 #define CF_PREC_SPLIT_FP16 1  /* MFMA complex GEMM, split hi+lo FP16 operands */
 
+/* execution plans (which passes and tiles a convolution of edge N runs) */
+#define CF_PLAN_AUTO 0       /* the autotuner's choice (default)                              */
+#define CF_PLAN_AXIS5 1      /* x, y, z*H*z, y^-1, x^-1: 11 volume transfers, every N          */
+#define CF_PLAN_PLANE3 2     /* xy-plane, z*H*z, xy-plane^-1: 7 transfers, N in {32,64,128}    */
+#define CF_PLAN_SPLIT3_G22 3 /* four-step y split, 7 transfers, N = 256, tile groups 2x2       */
+#define CF_PLAN_SPLIT3_G24 4 /*   ... tile groups 2x4                                          */
+#define CF_PLAN_SPLIT3_G42 5 /*   ... tile groups 4x2                                          */
+#define CF_PLAN_SPLIT3_G44 6 /*   ... tile groups 4x4                                          */
+#define CF_NUM_PLANS 7
+
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
 int cf_supported_size(unsigned N);
+
+/* 1 if `plan` can run edge N */
+int cf_plan_supported(unsigned N, int plan);
+/* Force the plan for edge N (CF_PLAN_AUTO returns the choice to the autotuner).
+ * Spectra already created keep the plan they were created for. */
+int cf_set_plan(unsigned N, int plan);
+/* The plan new spectra of edge N get; runs the autotuner on first use. */
+int cf_get_plan(unsigned N);
+/* Time every supported plan for edge N on scratch data (one cube, FP32) and
+ * make the fastest the automatic choice.  ms[CF_NUM_PLANS] (nullable) receives
+ * the mean device time per convolution of each plan, 0 for unsupported ones.
+ * The choice is also appended to the file named by $CONVFPGA_TUNE_FILE, if
+ * set, and read back from it by later processes. Returns the chosen plan. */
+int cf_autotune(unsigned N, float *ms);
 
 /* ---- precomputed filter spectra ("filter setup") ---------------------------
  * The reference recomputes the filter transform in every call
@@ -45,10 +69,13 @@ typedef struct cf_spectrum_s *cf_spectrum_t;
 int cf_spectrum_create(unsigned N, unsigned count, const void *filters, int on_device, void *stream,
                        cf_spectrum_t *out);
 void cf_spectrum_destroy(cf_spectrum_t h);
-/* device address of the spectra (count * N^3 complex floats, natural order) */
+/* device address of the spectra: count * N^3 complex floats, natural order for
+ * CF_PLAN_AXIS5 / CF_PLAN_PLANE3, an internal blocked order for CF_PLAN_SPLIT3_* */
 const void *cf_spectrum_data(cf_spectrum_t h);
 unsigned cf_spectrum_edge(cf_spectrum_t h);
 unsigned cf_spectrum_count(cf_spectrum_t h);
+/* the plan the spectrum was laid out for (its layout is internal to the plan) */
+int cf_spectrum_plan(cf_spectrum_t h);
 
 /* ---- device-resident convolution --------------------------------------------
  * how_many cubes d_sig -> d_out (may be the same buffer).  With a spectrum of

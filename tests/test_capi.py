@@ -108,3 +108,16 @@ def test_host_buffer_type_checks():
         C.fpgaf_conv3D(8, np.zeros(512, np.complex128), np.zeros(512, np.complex64), np.zeros(512, np.complex64))
     with pytest.raises(ValueError):
         C.fpgaf_conv3D(8, np.zeros(100, np.complex64), np.zeros(512, np.complex64), np.zeros(512, np.complex64))
+
+
+def test_plan_support_table():
+    L = api.lib()
+    for n in C.SUPPORTED_SIZES:
+        assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5) == 1
+        assert L.cf_plan_supported(n, api.CF_PLAN_PLANE3) == (1 if n in (32, 64, 128) else 0)
+        for p in range(api.CF_PLAN_SPLIT3_G22, api.CF_NUM_PLANS):
+            assert L.cf_plan_supported(n, p) == (1 if n == 256 else 0)
+    assert L.cf_plan_supported(1024, api.CF_PLAN_AXIS5) == 0
+    assert L.cf_plan_supported(256, 99) == 0
+    assert L.cf_set_plan(256, 99) == api.CF_ERR_ARG
+    assert L.cf_set_plan(256, api.CF_PLAN_SPLIT3_G44) == api.CF_ERR_STATE  # not initialised

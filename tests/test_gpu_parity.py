@@ -208,3 +208,92 @@ def test_device_fill_matches_host_generator(gpu, oracle):
     gpu.api.fill_uniform(x, 2 * x.numel(), 987654321)
     want = oracle.uniform(x.numel(), 987654321)
     assert np.array_equal(_np(x).view(np.uint32), want.view(np.uint32))
+
+
+PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 3), (256, 4), (256, 5), (256, 6),
+              (512, 1)]
+
+
+@pytest.mark.parametrize("n,plan", PLAN_CASES)
+def test_every_plan_matches_oracle(gpu, oracle, n, plan):
+    """Each schedule (5-pass axis, 3-pass plane, 3-pass four-step with its tile
+    variants) on a batch of 2 with per-instance spectra and with a shared one."""
+    import torch
+    print("case plan=%d N=%d" % (plan, n), flush=True)
+    api = gpu.api
+    api.set_plan(n, plan)
+    try:
+        b = 2 if n < 512 else 1
+        P = n ** 3
+        sig = oracle.uniform(b * P, 20000 + n + plan)
+        filt = oracle.uniform(b * P, 30000 + n + plan)
+        ds, df = _t(sig), _t(filt)
+        own = gpu.Spectrum(n, df, count=b)
+        shared = gpu.Spectrum(n, df[:P])
+        assert own.plan == plan and shared.plan == plan
+        o1, o2 = torch.empty_like(ds), torch.empty_like(ds)
+        gpu.conv3d_device(own, ds, o1, how_many=b)
+        gpu.conv3d_device(shared, ds, o2, how_many=b)
+        g1, g2 = _np(o1), _np(o2)
+        for i in range(b):
+            s = slice(i * P, (i + 1) * P)
+            si = np.ascontiguousarray(sig[s])
+            if n <= 256:
+                r1 = oracle.conv3d_f64(n, si, np.ascontiguousarray(filt[s]), 1.0 / P)
+                assert oracle.rel_l2(np.ascontiguousarray(g1[s]), r1) <= REL_L2_FP32
+            if i == 0 or n <= 128:
+                r2 = oracle.conv3d_f64(n, si, np.ascontiguousarray(filt[:P]), 1.0 / P)
+                assert oracle.rel_l2(np.ascontiguousarray(g2[s]), r2) <= REL_L2_FP32
+        # in-place call gives the same bits as out-of-place
+        gpu.conv3d_device(shared, ds, ds, how_many=b)
+        assert np.array_equal(_np(ds), g2)
+        del own, shared, ds, df, o1, o2
+        torch.cuda.empty_cache()
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+
+
+@pytest.mark.parametrize("plan", [1, 3, 6])
+def test_dropin_256_on_each_plan(gpu, oracle, plan):
+    n = 256
+    print("case drop-in N=256 plan=%d" % plan, flush=True)
+    gpu.api.set_plan(n, plan)
+    try:
+        sig, filt = oracle.uniform(n ** 3, 40001), oracle.uniform(n ** 3, 40002)
+        ref32 = oracle.conv3d_f32(n, sig, filt)
+        out = np.zeros_like(sig)
+        r = gpu.fpgaf_conv3D(n, sig, filt, out)
+        assert r.valid and min(r.filter_exec_t, r.sig_exec_t, r.siginv_exec_t, r.sig_pcie_wr_t, r.sig_pcie_rd_t) > 0
+        assert oracle.snr_db(ref32, out) > SNR_DB_MIN
+        out2 = np.zeros_like(sig)
+        r = gpu.fpgaf_conv3D_svm(n, sig, filt, out2)
+        assert r.valid and r.sig_pcie_wr_t == 0.0
+        assert oracle.snr_db(ref32, out2) > SNR_DB_MIN
+    finally:
+        gpu.api.set_plan(n, gpu.api.CF_PLAN_AUTO)
+
+
+@pytest.mark.parametrize("plan", [2, 5])
+def test_svm_batch_on_3pass_plans(gpu, oracle, plan):
+    n = 64 if plan == 2 else 256
+    hm = 3 if n == 64 else 2
+    print("case svm batch plan=%d N=%d" % (plan, n), flush=True)
+    gpu.api.set_plan(n, plan)
+    try:
+        sig = oracle.uniform(hm * n ** 3, 41000 + plan)
+        filt = oracle.uniform(n ** 3, 42000 + plan)
+        out = np.zeros_like(sig)
+        assert gpu.fpgaf_conv3D_svm_batch(n, sig, filt, out, hm).valid
+        assert oracle.snr_db(oracle.conv3d_f32(n, sig, filt, batch=hm), out) > SNR_DB_MIN
+    finally:
+        gpu.api.set_plan(n, gpu.api.CF_PLAN_AUTO)
+
+
+def test_autotuner_picks_a_supported_plan(gpu):
+    api = gpu.api
+    for n in (64, 256):
+        plan, ms = api.autotune(n)
+        print("autotune N=%d -> %s %s" % (n, api.PLAN_NAMES[plan], ms), flush=True)
+        assert api.plan_supported(n, plan)
+        assert plan in ms and ms[plan] == min(ms.values())
+        assert api.get_plan(n) == plan
