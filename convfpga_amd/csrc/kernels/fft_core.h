@@ -183,6 +183,13 @@ struct ColLds {
   __device__ __forceinline__ void st(int o, float2 x) const { p[o * (C + 1)] = x; }
   __device__ __forceinline__ float2 ld(int o) const { return p[o * (C + 1)]; }
 };
+// Column view with an explicit row pitch (in float2).
+template <int PITCH>
+struct PitchLds {
+  float2* p;  // already offset by the column
+  __device__ __forceinline__ void st(int o, float2 x) const { p[o * PITCH] = x; }
+  __device__ __forceinline__ float2 ld(int o) const { return p[o * PITCH]; }
+};
 struct RowLds {
   float2* p;  // start of this row's segment
   __device__ __forceinline__ void st(int o, float2 x) const { p[o + (o >> 4)] = x; }

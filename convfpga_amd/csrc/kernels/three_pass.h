@@ -120,12 +120,17 @@ struct Split256 {
   static constexpr int A_THREADS = 256 * GA;       // 16*GA rows of 16 threads (launch bound of xy1_*)
   static constexpr int A_TILES = N * (16 / GA);    // (z, y2 group) per cube
   static constexpr int A_ROW_PITCH = 272;          // 256 + 256/16
-  static constexpr int A_COL_PLANE = 256 * 17 + 4; // (kx, y1) block of one y2, padded
+  // (kx, y1) block of one y2: kx stride 17, planes offset so that the two y2
+  // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
+  static constexpr int A_COL_PLANE = 256 * 17 + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
   static constexpr int B_THREADS = 256 * GB;       // 16*GB z-lines of 16 threads (launch bound of mid)
   static constexpr int B_TILES = 16 * NXH;         // (k1, x block) per cube
   static constexpr int B_LINES = 16 * GB;          // z-lines per tile: (y2, x%GB)
-  static constexpr int B_LDS = 256 * (B_LINES + 1);
+  // row pitch 17*GB (= GB mod 32): the (kz, x%GB) reads of the y2 transpose
+  // step 2*GB banks per kz, so a half-wave touches every bank pair once
+  static constexpr int B_PITCH = 17 * GB;
+  static constexpr int B_LDS = 256 * B_PITCH;
   static constexpr int TILE = 4096 * GB;           // points per mid tile
   // intermediate index of (x, k1, y2, z)
   __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
@@ -209,9 +214,27 @@ struct Xy1Pass {
   }
 };
 
+// XCD-aware tile order.  Work-group b runs on XCD b % 8 and consecutive
+// work-groups of one XCD are dispatched together.  The S = 16/(GA*GB) tiles
+// of one z-plane whose y2 groups share 128-byte lines of the intermediate are
+// mapped to work-groups b, b+8, ..., b+8(S-1): one XCD, dispatched back to
+// back, so a line is fetched into (or assembled in) one L2 once instead of
+// once per tile.  A bijection on [0, ntotal) since ntotal is a multiple of 8*S.
+template <int S>
+__device__ __forceinline__ int xcd_group_tile(int b) {
+  if constexpr (S <= 1) {
+    return b;
+  } else {
+    const int xcd = b % 8, r = b / 8;
+    return ((r / S) * 8 + xcd) * S + r % S;
+  }
+}
+
 template <int GA, int GB, bool INV>
 __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
   using K = Split256<GA, GB>;
+  constexpr int S = (GA * GB >= 16) ? 1 : 16 / (GA * GB);
+  static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
   __shared__ float2 lds[K::A_LDS];
   __shared__ float2 tw_s[256];
   stage_twiddles<256, K::A_THREADS>(tw_s, tw);
@@ -219,7 +242,11 @@ __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const 
   const int row = x / 16;
   const int cxl = x % GB, cy2l = (x / GB) % GA, cxh = x / (GB * GA);
   const Xy1Pass<GA, GB, INV> k{in, out, tw_s, lds, scale, x % 16, row, row % 16, row / 16, cxh * GB + cxl, cy2l};
-  run_tiles(k, ntotal);
+  const int tile = xcd_group_tile<S>((int)blockIdx.x);
+  if (tile >= ntotal) return;
+  typename Xy1Pass<GA, GB, INV>::Regs a;
+  k.load(tile, a);
+  k.process(tile, a);
 }
 
 // Middle pass: one contiguous tile (k1, x block) = [z][y2][x%GB].
@@ -248,13 +275,14 @@ struct MidPass {
     for (int m = 0; m < 16; ++m) r.v[m] = src[K::B_THREADS * m];
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
-    const ColLds<L> view{lds + l};
+    constexpr int PITCH = K::B_PITCH;
+    const PitchLds<PITCH> view{lds + l};
     fft_line<N, false>(r.v, zt, view, tw_s);  // r.v[m] = Z(kz = zt + 16 m) of line l
 #pragma unroll
-    for (int m = 0; m < 16; ++m) lds[(zt + 16 * m) * (L + 1) + l] = r.v[m];
+    for (int m = 0; m < 16; ++m) lds[(zt + 16 * m) * PITCH + l] = r.v[m];
     __syncthreads();
     float2 u[16];
-    float2* rowp = lds + qz * (L + 1) + qx;
+    float2* rowp = lds + qz * PITCH + qx;
 #pragma unroll
     for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GB];
     const int cube = tile / K::B_TILES;
@@ -271,7 +299,7 @@ struct MidPass {
     for (int y2 = 0; y2 < 16; ++y2) rowp[y2 * GB] = u[y2];
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + 16 * m) * (L + 1) + l];
+    for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + 16 * m) * PITCH + l];
     __syncthreads();
     fft_line<N, true>(r.v, zt, view, tw_s);
     float2* dst = out + tile_off(tile) + threadIdx.x;
