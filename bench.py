@@ -2,12 +2,12 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2_256]
 
-A step is one convolution of one device-resident 256^3 complex64 signal with
-a filter whose spectrum was computed once before timing (the 256^3 single-cube
-configuration of the metric): forward x and y passes, the fused z pass
-(z forward, product with the spectrum, z inverse), inverse y and x passes
-with the 1/N^3 scale.  Inputs are synthetic (seeded U[-1,1) re/im, generated
-in HBM by the library).  Prints one JSON line.
+Default workload (the metric's configuration): a step is one convolution of
+one device-resident 256^3 complex64 signal with a filter whose spectrum was
+computed once before timing -- the schedule chosen by the library's
+autotuner (3-pass four-step or 5-pass axis-by-axis), ending with the 1/N^3
+scale.  Inputs are synthetic (seeded U[-1,1) re/im, generated in HBM by the
+library).  Prints one JSON line.
 
 Per-pass device times come from HIP events recorded by the library on the
 launch stream during the timed steps; the roofline entry is for the pass with
@@ -15,10 +15,16 @@ the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
 convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
 against the FP64 oracle.
+
+Other workloads (reported lines, not the metric): c2_128 / c1_64 (one cube),
+c3_64x64 (64 cubes of 64^3, one shared spectrum), c4_mixed (16x32^3 + 8x64^3 +
+4x128^3 + 1x256^3, own spectra, shuffled), each comparing the persistent
+one-launch kernel, the recorded graph and, where it applies, batched passes.
 """
 import argparse
 import json
 import os
+import random
 import statistics
 import sys
 import time
@@ -29,12 +35,10 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 
-WORKLOADS = {
-    # name: (N, batch, shared spectrum)
-    "c2_256": (256, 1, True),
-    "c2_128": (128, 1, True),
-    "c1_64": (64, 1, True),
-    "c3_64x64": (64, 64, True),
+SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64}
+BATCHES = {
+    "c3_64x64": [64] * 64,
+    "c4_mixed": [32] * 16 + [64] * 8 + [128] * 4 + [256],
 }
 
 
@@ -43,14 +47,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2_256", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2_256", choices=sorted(SINGLE) + sorted(BATCHES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--plan", type=int, default=0, help="force a schedule (CF_PLAN_*); 0 = autotuned")
+    ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
 
-def cpu_baseline_leg(n, batch, gpu_out_host, seed_sig, seed_filt, reps):
+def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
     """Times the oracle's FP32 CPU convolution (port of src/fft_conv3D.cpp:50-177)
     on `reps` convolutions of one cube, and checks the GPU output on the same
     inputs.  The only place bench.py uses the oracle."""
@@ -69,9 +74,8 @@ def cpu_baseline_leg(n, batch, gpu_out_host, seed_sig, seed_filt, reps):
         O.conv3d_f32_spec(n, sig, H, out)  # input re-copied every rep
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
-    threads = O.max_threads()
     res = {
-        "value": 1.0 / med, "unit": "conv/s", "cores": threads, "kind": "port",
+        "value": 1.0 / med, "unit": "conv/s", "cores": O.max_threads(), "kind": "port",
         "sample": "%d reps of one %d^3 FP32 convolution (FFT, product, inverse FFT; filter spectrum "
                   "precomputed), median; oracle's own radix-2 FFT with OpenMP (FFTW3 %s)"
                   % (reps, n, "present" if O.fftw_available() else "not installed"),
@@ -104,27 +108,26 @@ def copy_bandwidth(torch, nelem, reps=5):
     return 2.0 * 8 * nelem / (ms * 1e-3) / 1e9
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (no multi-GPU path); --gpus %d is not supported"
-              % a.gpus, file=sys.stderr)
-        return 2
-    import torch
-    import convfpga_amd as C
-    from convfpga_amd import api
+def base_line(a, metric, value, unit, ms_per_step, config):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32", "data": "synthetic (seeded U[-1,1) re/im generated on the GPU)", "config": config}
 
-    n, batch, shared = WORKLOADS[a.workload]
-    assert torch.cuda.is_available(), "no GPU"
-    torch.zeros(1, device="cuda")
-    rc = C.fpga_initialize("gfx950", C.CODE_OBJECT, False)
-    if rc != 0:
-        raise SystemExit("fpga_initialize -> %d" % rc)
+
+KERNEL_OF_PASS = {  # kernel name prefix of each pass name, for the traffic lookup
+    "xy1_fwd": "cf_xy1_fwd_256", "xy1_inv": "cf_xy1_inv_256", "y2z_fwd_mul_inv": "cf_mid_256",
+    "x_fwd": "cf_row_fwd", "x_inv": "cf_row_inv", "y_fwd": "cf_coly_fwd", "y_inv": "cf_coly_inv",
+    "z_fwd_mul_z_inv": "cf_colz_conv", "xy_fwd": "cf_plane_fwd", "xy_inv": "cf_plane_inv",
+}
+
+
+def run_single(a, torch, C, api):
+    n = SINGLE[a.workload]
     P = n ** 3
-    seed_sig, seed_filt = 1, 2
-    sig = torch.empty(batch * P, dtype=torch.complex64, device="cuda")
+    seed_sig, seed_filt = a.seed, a.seed + 1
+    sig = torch.empty(P, dtype=torch.complex64, device="cuda")
     filt = torch.empty(P, dtype=torch.complex64, device="cuda")
-    api.fill_uniform(sig, 2 * P * batch, seed_sig)  # one stream over the batch: every cube differs
+    api.fill_uniform(sig, 2 * P, seed_sig)
     api.fill_uniform(filt, 2 * P, seed_filt)
     torch.cuda.synchronize()
     if a.plan:
@@ -135,64 +138,161 @@ def main():
     spec = C.Spectrum(n, filt)  # filter setup, outside the timed region
     assert spec.plan == plan
     out = torch.empty_like(sig)
-
     # plain copy kernel of the same footprint: measured HBM bandwidth beside the
     # datasheet peak, and the calibration dispatch for the HBM byte counters
-    copy_gbs = copy_bandwidth(torch, max(batch * P, 1 << 24))
+    copy_gbs = copy_bandwidth(torch, max(P, 1 << 24))
 
     for _ in range(a.warmup):
-        C.conv3d_device(spec, sig, out, how_many=batch)
+        C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
     api.profile_begin(a.steps)
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        C.conv3d_device(spec, sig, out, how_many=batch)
+        C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     passes, ncalls = api.profile_end()
     assert ncalls == a.steps
-
     wall = t1 - t0
-    ms_per_step = 1e3 * wall / a.steps
-    value = a.steps * batch / wall
-    V = 8.0 * P
     alg_bytes = sum(p["bytes"] for p in passes)  # per step
     for p in passes:
         p["GBps"] = p["bytes"] / (p["ms"] * 1e-3) / 1e9
         p["frac_of_peak"] = p["GBps"] / HBM_PEAK_GBS
     dom = max(passes, key=lambda p: p["ms"])
-    dev_ms = sum(p["ms"] for p in passes)
+    kprefix = KERNEL_OF_PASS.get(dom["name"], "")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_256.json")
+    if n == 256 and kprefix and os.path.exists(tfile):
+        for k, v in json.load(open(tfile)).items():
+            if v.get("plan") == api.PLAN_NAMES[plan] and k.startswith(kprefix):
+                traffic = v["traffic_bytes"]
+    line = base_line(a, "3D convs/sec + % HBM roofline at %d^3 c2c" % n, a.steps / wall, "conv/s",
+                     1e3 * wall / a.steps,
+                     {"workload": "conv3d_%d^3_c2c_fp32_batch1_precomputed_filter_spectrum" % n, "N": n, "batch": 1,
+                      "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
+                      "algorithmic_bytes_per_step": alg_bytes})
+    line["roofline"] = {"bound": "hbm", "kernel": kprefix, "pass": dom["name"], "achieved": dom["GBps"],
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic}
+    line["conv_roofline_frac"] = alg_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS
+    # the same rate priced with the 5-pass model: 11 volume transfers per convolution
+    line["conv_roofline_frac_5pass_model"] = 11 * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS
+    line["device_ms_per_step"] = sum(p["ms"] for p in passes)
+    line["copy_kernel_GBps"] = copy_gbs
+    line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
+    line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
+    host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
+    return line, (n, host_out, seed_sig, seed_filt)
 
-    line = {
-        "metric": "3D convs/sec + % HBM roofline at 256^3 c2c",
-        "value": value,
-        "unit": "conv/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded U[-1,1) re/im generated on the GPU)",
-        "config": {"workload": "conv3d_%d^3_c2c_fp32_batch%d_precomputed_filter_spectrum" % (n, batch),
-                   "N": n, "batch": batch, "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
-                   "algorithmic_bytes_per_step": alg_bytes},
-        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
-        "conv_roofline_frac": alg_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS,
-        # the same rate priced with the 5-pass model (11 volume transfers per
-        # convolution with its own spectrum, 2*5 per batched cube + 1)
-        "conv_roofline_frac_5pass_model": (2 * 5 * batch + 1) * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS,
-        "device_ms_per_step": dev_ms,
-        "copy_kernel_GBps": copy_gbs,
-        "autotune_ms_per_conv": {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()},
-        "passes": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes],
-    }
-    if not a.no_cpu_baseline:
-        host_out = out[:P].cpu().numpy()
-        base, check = cpu_baseline_leg(n, batch, host_out, seed_sig, seed_filt, a.cpu_reps)
+
+def run_batch(a, torch, C, api):
+    sizes = list(BATCHES[a.workload])
+    shared = a.workload == "c3_64x64"
+    if not shared:
+        random.Random(a.seed).shuffle(sizes)  # instance order shuffled by seed
+    by_n = {}
+    for n in sizes:
+        by_n[n] = by_n.get(n, 0) + 1
+    # signals / outputs of one edge are contiguous
+    sig, out, filt = {}, {}, {}
+    for n, cnt in by_n.items():
+        P = n ** 3
+        sig[n] = torch.empty(cnt * P, dtype=torch.complex64, device="cuda")
+        out[n] = torch.empty_like(sig[n])
+        filt[n] = torch.empty((1 if shared else cnt) * P, dtype=torch.complex64, device="cuda")
+        api.fill_uniform(sig[n], 2 * cnt * P, a.seed + n)
+        api.fill_uniform(filt[n], filt[n].numel() * 2, a.seed + 7 * n)
+    torch.cuda.synchronize()
+    # spectra in natural order (the persistent kernel's layout): axis plan for 256
+    if 256 in by_n:
+        api.set_plan(256, api.CF_PLAN_AXIS5)
+    specs = {}
+    for n, cnt in by_n.items():
+        P = n ** 3
+        specs[n] = [C.Spectrum(n, filt[n][k * P:(k + 1) * P]) for k in range(1 if shared else cnt)]
+    used = {n: 0 for n in by_n}
+    inst = []
+    for n in sizes:  # in the (shuffled) instance order
+        k = used[n]
+        used[n] += 1
+        P = n ** 3
+        inst.append((sig[n][k * P:(k + 1) * P], out[n][k * P:(k + 1) * P], specs[n][0 if shared else k]))
+    # graph mode records contiguous runs of one spectrum as batched passes: order by edge
+    inst_sorted = sorted(inst, key=lambda t: (t[2].N, t[0].data_ptr()))
+    results = {}
+
+    def timed(launch, sync_check):
+        for _ in range(a.warmup):
+            launch()
+        sync_check()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            launch()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sync_check()
+        return (t1 - t0) / a.steps
+
+    bp = api.Batch([(s, o, h) for s, o, h in inst], mode=api.CF_BATCH_PERSISTENT)
+    results["persistent_one_launch"] = timed(bp.launch, bp.check)
+    bg = api.Batch([(s, o, h) for s, o, h in inst_sorted], mode=api.CF_BATCH_GRAPH)
+    results["graph"] = timed(bg.launch, bg.check)
+    if shared:  # the batch as one sequence of batched passes of the autotuned plan
+        sp = C.Spectrum(64, filt[64][:64 ** 3])
+        results["batched_passes_" + api.PLAN_NAMES[sp.plan]] = timed(
+            lambda: C.conv3d_device(sp, sig[64], out[64], how_many=len(sizes)), torch.cuda.synchronize)
+    per_size = {}
+    if not shared:  # per-edge throughput: each edge group alone in one persistent launch
+        for n, cnt in sorted(by_n.items()):
+            b = api.Batch([(s, o, h) for s, o, h in inst if h.N == n], mode=api.CF_BATCH_PERSISTENT)
+            w = timed(b.launch, b.check)
+            per_size["%d^3 x%d" % (n, cnt)] = {"conv_per_s": cnt / w, "ms": 1e3 * w}
+            b.close()
+    best = min(results, key=lambda k: results[k])
+    wall = results[best]
+    if shared:
+        alg = (2 * 5 * len(sizes) + 1) * 8.0 * 64 ** 3
+    else:
+        alg = sum((2 * 5 + 1) * 8.0 * n ** 3 for n in sizes)
+    line = base_line(a, "batched 3D convs/sec (%s)" % a.workload, len(sizes) / wall, "conv/s", 1e3 * wall,
+                     {"workload": a.workload, "instances": len(sizes),
+                      "edges": {str(n): c for n, c in sorted(by_n.items())},
+                      "filters": "one shared" if shared else "own per instance", "mode": best, "scale": "1/N^3"})
+    line["modes_ms_per_batch"] = {k: round(1e3 * v, 4) for k, v in results.items()}
+    line["roofline"] = {"bound": "hbm", "kernel": "cf_batch_run" if best.startswith("persistent") else best,
+                        "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / wall / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_model": "5-pass algorithmic bytes of the whole batch"}
+    if per_size:
+        line["per_size"] = per_size
+    bp.close()
+    bg.close()
+    if 256 in by_n:
+        api.set_plan(256, api.CF_PLAN_AUTO)
+    return line, None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); --gpus %d is not supported"
+              % a.gpus, file=sys.stderr)
+        return 2
+    import torch
+    import convfpga_amd as C
+    from convfpga_amd import api
+
+    assert torch.cuda.is_available(), "no GPU"
+    torch.zeros(1, device="cuda")
+    rc = C.fpga_initialize("gfx950", C.CODE_OBJECT, False)
+    if rc != 0:
+        raise SystemExit("fpga_initialize -> %d" % rc)
+    if a.workload in SINGLE:
+        line, cpu = run_single(a, torch, C, api)
+    else:
+        line, cpu = run_batch(a, torch, C, api)
+    if cpu is not None and not a.no_cpu_baseline:
+        base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps)
         line["cpu_baseline"] = base
         line.update(check)
     else:

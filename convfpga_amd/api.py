@@ -28,6 +28,7 @@ CF_MAX_PASSES = 8
 CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
 CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
 CF_NUM_PLANS = 7
+CF_BATCH_PERSISTENT, CF_BATCH_GRAPH = 0, 1
 PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
@@ -38,7 +39,8 @@ EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_s
                "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_conv3d_device",
                "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
-               "cf_autotune", "cf_spectrum_plan")
+               "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
+               "cf_batch_destroy")
 
 
 class fpga_t(ctypes.Structure):
@@ -60,6 +62,10 @@ class cf_pass_times_t(ctypes.Structure):
     def as_list(self):
         return [{"name": self.name[i].value.decode(), "ms": float(self.ms[i]), "bytes": float(self.bytes[i])}
                 for i in range(self.npass)]
+
+
+class cf_instance_t(ctypes.Structure):
+    _fields_ = [("d_sig", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("h", ctypes.c_void_p)]
 
 
 class ConvError(RuntimeError):
@@ -102,6 +108,10 @@ def lib():
             "cf_get_plan": (i, [u]),
             "cf_autotune": (i, [u, ctypes.POINTER(ctypes.c_float)]),
             "cf_spectrum_plan": (i, [vp]),
+            "cf_batch_create": (i, [ctypes.POINTER(cf_instance_t), u, i, i, ctypes.POINTER(vp)]),
+            "cf_batch_launch": (i, [vp, vp]),
+            "cf_batch_check": (i, [vp]),
+            "cf_batch_destroy": (None, [vp]),
             "cf_profile_end": (i, [ctypes.POINTER(cf_pass_times_t), ctypes.POINTER(u)]),
         }
         for name, (res, args) in sig.items():
@@ -290,3 +300,33 @@ def autotune(N):
     if rc < 0:
         _check(rc, "cf_autotune")
     return rc, {p: ms[p] for p in range(1, CF_NUM_PLANS) if ms[p] > 0}
+
+
+class Batch:
+    """A fixed list of (signal, output, Spectrum) convolutions replayed with one
+    graph launch (cf_batch_t).  Keeps references to the spectra it uses."""
+
+    def __init__(self, instances, scale=CF_SCALE_INV_N3, mode=CF_BATCH_PERSISTENT):
+        arr = (cf_instance_t * len(instances))()
+        for k, (sig, out, spec) in enumerate(instances):
+            arr[k] = cf_instance_t(_dev(sig), _dev(out), spec.handle)
+        h = ctypes.c_void_p()
+        _check(lib().cf_batch_create(arr, len(instances), scale, mode, ctypes.byref(h)), "cf_batch_create")
+        self._h, self._keep, self.mode = h, list(instances), mode
+
+    def launch(self, stream=None):
+        _check(lib().cf_batch_launch(self._h, _stream(stream)), "cf_batch_launch")
+
+    def check(self):
+        _check(lib().cf_batch_check(self._h), "cf_batch_check")
+
+    def close(self):
+        if self._h:
+            lib().cf_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

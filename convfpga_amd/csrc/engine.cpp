@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_desc.h"
 #include "convfpga/convfpga_ext.h"
 #include "engine.h"
 #include "geometry.h"
@@ -86,6 +87,8 @@ struct Engine {
   hipModule_t mod = nullptr;
   hipStream_t stream = nullptr;
   hipFunction_t twgen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr;
+  hipFunction_t batch_run = nullptr, batch_reset = nullptr;
+  hipStream_t capture = nullptr;  // private stream for graph capture
   std::map<unsigned, SizeFns> sizes;
   // workspace of the host-buffer (drop-in) calls
   void* d_data = nullptr;
@@ -162,7 +165,9 @@ void release_all() {
   g.stream = nullptr;
   if (g.mod) hipModuleUnload(g.mod);
   g.mod = nullptr;
-  g.twgen = g.errp = g.errf = g.fillu = nullptr;
+  g.twgen = g.errp = g.errf = g.fillu = g.batch_run = g.batch_reset = nullptr;
+  if (g.capture) hipStreamDestroy(g.capture);
+  g.capture = nullptr;
   g.ready = false;
   g.svm = false;
 }
@@ -561,6 +566,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
   }
   if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
       get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
+      get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_reset, "cf_batch_reset") != CF_OK ||
       ensure_events() != CF_OK) {
     fprintf(stderr, "convfpga: '%s' is not a convfpga code object (%s)\n", path, g_err.c_str());
     release_all();
@@ -915,5 +921,192 @@ extern "C" int cf_fill_uniform(void* d_out, size_t nfloats, unsigned long long s
   void* args[] = {&d_out, &nf, &sd};
   HIPCHK(hipModuleLaunchKernel(g.fillu, blocks, 1, 1, 256, 1, 1, 0, (hipStream_t)stream, args, nullptr));
   return CF_OK;
+}
+// ---- batches -------------------------------------------------------------------
+struct cf_batch_s {
+  int mode = CF_BATCH_PERSISTENT;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* d_segs = nullptr;
+  unsigned* d_ctr = nullptr;
+  void* d_work = nullptr;  // own workspace of the graph mode (four-step plans)
+  hipStream_t last = nullptr;
+  bool launched = false;
+};
+
+static void batch_free(cf_batch_s* b) {
+  if (!b) return;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->d_segs) hipFree(b->d_segs);
+  if (b->d_ctr) hipFree(b->d_ctr);
+  if (b->d_work) hipFree(b->d_work);
+  delete b;
+}
+
+static int batch_capture_end(cf_batch_s* b, int rc) {
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(g.capture, &graph);
+  if (rc != CF_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(CF_ERR_RUNTIME, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = graph;
+  HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+  return CF_OK;
+}
+
+extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int scale_mode, int mode, cf_batch_t* out) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!inst || !out || count == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  *out = nullptr;
+  if (scale_mode != CF_SCALE_NONE && scale_mode != CF_SCALE_INV_N3) return fail(CF_ERR_ARG, "bad scale_mode");
+  if (mode != CF_BATCH_PERSISTENT && mode != CF_BATCH_GRAPH) return fail(CF_ERR_ARG, "bad batch mode %d", mode);
+  CFCHK(require_ready());
+  std::vector<SizeFns*> fns(count);
+  size_t work_need = 0;
+  for (unsigned i = 0; i < count; ++i) {
+    const cf_instance_t& x = inst[i];
+    if (!x.d_sig || !x.d_out || !x.h) return fail(CF_ERR_ARG, "instance %u: NULL pointer", i);
+    if (x.h->count != 1) return fail(CF_ERR_ARG, "instance %u: spectrum must hold one filter", i);
+    CFCHK(size_res(x.h->n, &fns[i]));
+    if (mode == CF_BATCH_PERSISTENT) {
+      if (x.h->n < 16 || x.h->n > 256)
+        return fail(CF_ERR_SIZE, "instance %u: the persistent batch runs edges 16..256, not %u", i, x.h->n);
+      if (is_split(x.h->plan))
+        return fail(CF_ERR_ARG, "instance %u: spectrum laid out for a four-step plan; create it under "
+                    "CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
+    } else if (is_split(x.h->plan)) {
+      work_need = std::max(work_need, cube_bytes(x.h->n));
+    }
+  }
+  if (!g.capture) HIPCHK(hipStreamCreateWithFlags(&g.capture, hipStreamNonBlocking));
+  auto* b = new cf_batch_s;
+  b->mode = mode;
+  int rc = CF_OK;
+  if (mode == CF_BATCH_PERSISTENT) {
+    // segments: pass-major, instance-minor
+    std::vector<CfBatchSeg> segs;
+    int item = 0;
+    for (int s = 0; s < kBatchKinds; ++s)
+      for (unsigned i = 0; i < count; ++i) {
+        const unsigned n = inst[i].h->n;
+        const bool row = s == 0 || s == 4;
+        CfBatchSeg sg{};
+        sg.in = s == 0 ? inst[i].d_sig : inst[i].d_out;
+        sg.out = inst[i].d_out;
+        sg.H = inst[i].h->d;
+        sg.tw = fns[i]->tw;
+        sg.scale = (s == 4 && scale_mode == CF_SCALE_INV_N3) ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+        sg.n = (int)n;
+        sg.kind = s;
+        sg.first_item = item;
+        sg.ntiles = row ? cfgeo::row_tiles((int)n) : cfgeo::col_tiles((int)n);
+        sg.dep = s == 0 ? -1 : (int)((s - 1) * count + i);
+        sg.dep_need = s == 0 ? 0 : segs[(size_t)(s - 1) * count + i].ntiles;
+        item += sg.ntiles;
+        segs.push_back(sg);
+      }
+    unsigned long long spin_limit = 1ull << 21;
+    if (const char* t = getenv("CONVFPGA_TEST_BATCH_STALL")) {
+      // test hook: make the last dependency unsatisfiable and the wait short, so
+      // the error path runs to completion without a long stall
+      if (atoi(t) == 1) {
+        segs.back().dep_need += 1;
+        spin_limit = 4096;
+      }
+    }
+    const int nsegs = (int)segs.size(), nitems = item, nctr = 2 + nsegs;
+    hipError_t e = hipMalloc(&b->d_segs, sizeof(CfBatchSeg) * segs.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctr, sizeof(unsigned) * nctr);
+    if (e == hipSuccess) e = hipMemcpy(b->d_segs, segs.data(), sizeof(CfBatchSeg) * segs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      batch_free(b);
+      return fail(CF_ERR_NOMEM, "batch descriptor allocation: %s", hipGetErrorString(e));
+    }
+    int per_cu = 1, cus = 1;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.batch_run, kBatchThreads, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device) != hipSuccess || cus < 1) cus = 1;
+    const unsigned grid = (unsigned)std::min(nitems, per_cu * cus);
+    HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
+    unsigned* ctr = b->d_ctr;
+    int nc = nctr;
+    void* ra[] = {&ctr, &nc};
+    const void* sp = b->d_segs;
+    int ns = nsegs, ni = nitems;
+    void* ba[] = {(void*)&sp, &ns, &ni, &ctr, &spin_limit};
+    if (hipModuleLaunchKernel(g.batch_reset, 1, 1, 1, 256, 1, 1, 0, g.capture, ra, nullptr) != hipSuccess ||
+        hipModuleLaunchKernel(g.batch_run, grid, 1, 1, kBatchThreads, 1, 1, 0, g.capture, ba, nullptr) != hipSuccess)
+      rc = fail(CF_ERR_RUNTIME, "batch kernel capture failed");
+    rc = batch_capture_end(b, rc);
+  } else {
+    if (work_need) {
+      const hipError_t e = hipMalloc(&b->d_work, work_need);
+      if (e != hipSuccess) {
+        batch_free(b);
+        return fail(CF_ERR_NOMEM, "batch workspace: %s", hipGetErrorString(e));
+      }
+    }
+    HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
+    // runs of consecutive instances with one edge, one spectrum and contiguous
+    // signals / outputs are recorded as one batched pass sequence
+    for (unsigned i = 0; i < count && rc == CF_OK;) {
+      const unsigned n = inst[i].h->n;
+      const size_t V = cube_bytes(n);
+      unsigned run = 1;
+      while (i + run < count && inst[i + run].h == inst[i].h &&
+             (const char*)inst[i + run].d_sig == (const char*)inst[i].d_sig + run * V &&
+             (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V && !is_split(inst[i].h->plan))
+        ++run;
+      const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+      std::vector<Pass> p;
+      conv_passes(*fns[i], inst[i].h->plan, n, run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, 0, scale,
+                  p);
+      rc = run_passes(p, n, run, g.capture, fns[i]->tw, nullptr);
+      i += run;
+    }
+    rc = batch_capture_end(b, rc);
+  }
+  if (rc != CF_OK) {
+    batch_free(b);
+    return rc;
+  }
+  *out = b;
+  return CF_OK;
+}
+
+extern "C" int cf_batch_launch(cf_batch_t b, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!b || !b->exec) return fail(CF_ERR_ARG, "NULL batch");
+  CFCHK(require_ready());
+  HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  b->last = (hipStream_t)stream;
+  b->launched = true;
+  return CF_OK;
+}
+
+extern "C" int cf_batch_check(cf_batch_t b) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!b) return fail(CF_ERR_ARG, "NULL batch");
+  CFCHK(require_ready());
+  if (!b->launched) return CF_OK;
+  HIPCHK(hipStreamSynchronize(b->last));
+  if (b->mode == CF_BATCH_PERSISTENT) {
+    unsigned flag = 0;
+    HIPCHK(hipMemcpy(&flag, b->d_ctr + 1, sizeof flag, hipMemcpyDeviceToHost));
+    if (flag) return fail(CF_ERR_RUNTIME, "persistent batch: a pass dependency was not met in time");
+  }
+  return CF_OK;
+}
+
+extern "C" void cf_batch_destroy(cf_batch_t b) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!b) return;
+  hipSetDevice(g.device);
+  if (b->launched) hipStreamSynchronize(b->last);
+  batch_free(b);
 }
 #pragma GCC visibility pop

@@ -11,6 +11,7 @@
 // passes that multiply by it (cube b uses H + b*hstride), tw the FP32 twiddle
 // table W_N^j, scale the factor of the last inverse pass.
 #include "axis_passes.h"
+#include "batch_kernel.h"
 #include "fft_core.h"
 #include "three_pass.h"
 #include "../geometry.h"
@@ -100,6 +101,18 @@ struct SplitGeoCheck {
   static constexpr bool ok = true;
 };
 static_assert(SplitGeoCheck<0>::ok && SplitGeoCheck<1>::ok && SplitGeoCheck<2>::ok && SplitGeoCheck<3>::ok, "");
+
+// ---- persistent batch of mixed sizes (batch_kernel.h) -----------------------------
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(
+    const CfBatchSeg* segs, int nsegs, int nitems, unsigned* ctr, unsigned long long spin_limit) {
+  cfk::batch_kernel(segs, nsegs, nitems, ctr, spin_limit);
+}
+
+// Zeroes the queue head, error flag and completion counters (a kernel rather
+// than a memset so that it can sit in a captured graph).
+extern "C" __global__ __launch_bounds__(256) void cf_batch_reset(unsigned* ctr, int n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) ctr[i] = 0u;
+}
 
 // ---- FP64 table generation -------------------------------------------------
 // tw[j] = exp(-2 pi i j / n) evaluated in double (sincospi of the exact

@@ -102,6 +102,31 @@ int cf_conv3d_device_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsi
 int cf_profile_begin(unsigned max_calls);
 int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
 
+/* ---- batches of mixed sizes ------------------------------------------------
+ * A fixed list of convolutions (each: device signal, device output, spectrum
+ * of count 1 -- instances may share a spectrum) prepared once and replayed
+ * with one graph launch per call.
+ *   CF_BATCH_PERSISTENT: one persistent kernel runs every pass tile of every
+ *     instance from a single work queue (pass-major order, per-instance pass
+ *     dependencies); edges 16..256, spectra of the natural-order plans
+ *     (CF_PLAN_AXIS5 / CF_PLAN_PLANE3).
+ *   CF_BATCH_GRAPH: the passes of each instance's own plan, recorded in order
+ *     into one graph (any edge and plan).
+ * Outputs may alias their signals.  Concurrent launches of one batch race. */
+typedef struct {
+  const void *d_sig;
+  void *d_out;
+  cf_spectrum_t h;
+} cf_instance_t;
+typedef struct cf_batch_s *cf_batch_t;
+#define CF_BATCH_PERSISTENT 0
+#define CF_BATCH_GRAPH 1
+int cf_batch_create(const cf_instance_t *inst, unsigned count, int scale_mode, int mode, cf_batch_t *out);
+int cf_batch_launch(cf_batch_t b, void *stream);
+/* waits for the last launch; CF_ERR_RUNTIME if the persistent kernel reported a stalled dependency */
+int cf_batch_check(cf_batch_t b);
+void cf_batch_destroy(cf_batch_t b);
+
 /* ---- plain 3D transforms (device) ------------------------------------------ */
 /* forward (inverse = 0) or unnormalised inverse (inverse = 1) 3D FFT */
 int cf_fft3d_device(unsigned N, const void *d_in, void *d_out, unsigned how_many, int inverse, void *stream);

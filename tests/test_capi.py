@@ -121,3 +121,23 @@ def test_plan_support_table():
     assert L.cf_plan_supported(256, 99) == 0
     assert L.cf_set_plan(256, 99) == api.CF_ERR_ARG
     assert L.cf_set_plan(256, api.CF_PLAN_SPLIT3_G44) == api.CF_ERR_STATE  # not initialised
+
+
+def test_batch_api_argument_checks():
+    L = api.lib()
+    h = ctypes.c_void_p()
+    assert L.cf_batch_create(None, 1, 0, 0, ctypes.byref(h)) == api.CF_ERR_ARG
+    inst = (api.cf_instance_t * 1)()
+    assert L.cf_batch_create(inst, 0, 0, 0, ctypes.byref(h)) == api.CF_ERR_ARG
+    assert L.cf_batch_create(inst, 1, 7, 0, ctypes.byref(h)) == api.CF_ERR_ARG
+    assert L.cf_batch_create(inst, 1, 0, 9, ctypes.byref(h)) == api.CF_ERR_ARG
+    assert L.cf_batch_create(inst, 1, 0, 0, ctypes.byref(h)) == api.CF_ERR_STATE  # not initialised
+    assert L.cf_batch_launch(None, None) == api.CF_ERR_ARG
+    assert L.cf_batch_check(None) == api.CF_ERR_ARG
+    L.cf_batch_destroy(None)
+
+
+def test_batch_descriptor_is_64_bytes():
+    # CfBatchSeg (convfpga_amd/csrc/batch_desc.h) is shared by host and device code
+    src = open(os.path.join(ROOT, "convfpga_amd", "csrc", "batch_desc.h")).read()
+    assert "sizeof(CfBatchSeg) == 64" in src

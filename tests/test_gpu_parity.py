@@ -297,3 +297,102 @@ def test_autotuner_picks_a_supported_plan(gpu):
         assert api.plan_supported(n, plan)
         assert plan in ms and ms[plan] == min(ms.values())
         assert api.get_plan(n) == plan
+
+
+def _mixed_instances(gpu, oracle, sizes, plan_for_256):
+    """(sig, out, spectrum, host sig, host filt) per instance, each with its own filter."""
+    api = gpu.api
+    inst = []
+    for k, n in enumerate(sizes):
+        if n == 256:
+            api.set_plan(256, plan_for_256)
+        s = oracle.uniform(n ** 3, 50000 + 10 * k)
+        f = oracle.uniform(n ** 3, 60000 + 10 * k)
+        spec = gpu.Spectrum(n, _t(f))
+        if n == 256:
+            api.set_plan(256, api.CF_PLAN_AUTO)
+        ds = _t(s)
+        inst.append((ds, ds.clone().zero_(), spec, s, f))
+    return inst
+
+
+def _check_instances(oracle, inst):
+    for ds, do, spec, s, f in inst:
+        n = spec.N
+        got = _np(do)
+        ref = oracle.conv3d_f64(n, s, f, 1.0 / n ** 3)
+        assert oracle.rel_l2(got, ref) <= REL_L2_FP32, n
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_mixed_sizes_replayed(gpu, oracle, mode):
+    """Mixed edges with their own filters in one graph launch (persistent kernel
+    or recorded per-instance passes); replayed with eager work in between, every
+    replay checked."""
+    import torch
+    api = gpu.api
+    sizes = [32, 64, 16, 128, 32, 256, 64, 32]
+    print("case batch mode=%d sizes=%s" % (mode, sizes), flush=True)
+    plan256 = api.CF_PLAN_AXIS5 if mode == api.CF_BATCH_PERSISTENT else api.CF_PLAN_SPLIT3_G24
+    inst = _mixed_instances(gpu, oracle, sizes, plan256)
+    b = gpu.api.Batch([(ds, do, spec) for ds, do, spec, _, _ in inst], mode=mode)
+    for rep in range(3):
+        for _, do, _, _, _ in inst:
+            do.zero_()
+        b.launch()
+        b.check()
+        _check_instances(oracle, inst)
+        # eager work on the same device between replays
+        x = torch.ones(1 << 20, device="cuda")
+        assert float((x * 2).sum()) == 2.0 * (1 << 20)
+    b.close()
+
+
+def test_batch_shared_spectrum_in_place(gpu, oracle):
+    n, k = 64, 4
+    print("case batch shared spectrum in place", flush=True)
+    f = oracle.uniform(n ** 3, 70001)
+    spec = gpu.Spectrum(n, _t(f))
+    sigs = [oracle.uniform(n ** 3, 70100 + i) for i in range(k)]
+    ds = [_t(s) for s in sigs]
+    b = gpu.api.Batch([(d, d, spec) for d in ds])
+    b.launch()
+    b.check()
+    for d, s in zip(ds, sigs):
+        assert oracle.rel_l2(_np(d), oracle.conv3d_f64(n, s, f, 1.0 / n ** 3)) <= REL_L2_FP32
+    b.close()
+
+
+def test_batch_stalled_dependency_is_reported(gpu, oracle, monkeypatch):
+    """The persistent kernel's bounded wait: a dependency made unsatisfiable
+    from the host (with a short spin limit) ends the launch with an error
+    instead of a hang; a batch built afterwards runs normally."""
+    print("case batch forced stall", flush=True)
+    n = 32
+    f = oracle.uniform(n ** 3, 71001)
+    spec = gpu.Spectrum(n, _t(f))
+    d = _t(oracle.uniform(n ** 3, 71002))
+    monkeypatch.setenv("CONVFPGA_TEST_BATCH_STALL", "1")
+    b = gpu.api.Batch([(d, d.clone(), spec)])
+    monkeypatch.delenv("CONVFPGA_TEST_BATCH_STALL")
+    b.launch()
+    with pytest.raises(gpu.ConvError, match="dependency"):
+        b.check()
+    b.close()
+    out = d.clone()
+    ok = gpu.api.Batch([(d, out, spec)])
+    ok.launch()
+    ok.check()
+    ok.close()
+
+
+def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle):
+    api = gpu.api
+    api.set_plan(256, api.CF_PLAN_SPLIT3_G24)
+    try:
+        spec = gpu.Spectrum(256, _t(oracle.uniform(256 ** 3, 72001)))
+    finally:
+        api.set_plan(256, api.CF_PLAN_AUTO)
+    d = _t(oracle.uniform(256 ** 3, 72002))
+    with pytest.raises(gpu.ConvError, match="four-step"):
+        api.Batch([(d, d, spec)], mode=api.CF_BATCH_PERSISTENT)
