@@ -166,7 +166,7 @@ def run_single(a, torch, C, api):
         for k, v in json.load(open(tfile)).items():
             if v.get("plan") == api.PLAN_NAMES[plan] and k.startswith(kprefix):
                 traffic = v["traffic_bytes"]
-    line = base_line(a, "3D convs/sec + % HBM roofline at %d^3 c2c" % n, a.steps / wall, "conv/s",
+    line = base_line(a, "3D convs/sec + %% HBM roofline at %d^3 c2c" % n, a.steps / wall, "conv/s",
                      1e3 * wall / a.steps,
                      {"workload": "conv3d_%d^3_c2c_fp32_batch1_precomputed_filter_spectrum" % n, "N": n, "batch": 1,
                       "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
@@ -250,10 +250,20 @@ def run_batch(a, torch, C, api):
             b.close()
     best = min(results, key=lambda k: results[k])
     wall = results[best]
-    if shared:
-        alg = (2 * 5 * len(sizes) + 1) * 8.0 * 64 ** 3
+
+    def npass(plan):
+        return 5 if plan == api.CF_PLAN_AXIS5 else 3
+
+    # algorithmic bytes of the executed schedule: 2 volumes per pass per cube,
+    # plus one spectrum read per distinct spectrum
+    if best.startswith("persistent"):
+        passes_of = {id(h): 5 for _, _, h in inst}
+    elif best.startswith("batched"):
+        passes_of = {id(h): npass(sp.plan) for _, _, h in inst}
     else:
-        alg = sum((2 * 5 + 1) * 8.0 * n ** 3 for n in sizes)
+        passes_of = {id(h): npass(h.plan) for _, _, h in inst}
+    alg = sum(2 * passes_of[id(h)] * 8.0 * h.N ** 3 for _, _, h in inst)
+    alg += sum(8.0 * h.N ** 3 for h in {id(h): h for _, _, h in inst}.values())
     line = base_line(a, "batched 3D convs/sec (%s)" % a.workload, len(sizes) / wall, "conv/s", 1e3 * wall,
                      {"workload": a.workload, "instances": len(sizes),
                       "edges": {str(n): c for n, c in sorted(by_n.items())},
@@ -262,7 +272,7 @@ def run_batch(a, torch, C, api):
     line["roofline"] = {"bound": "hbm", "kernel": "cf_batch_run" if best.startswith("persistent") else best,
                         "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": alg / wall / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                        "bytes_model": "5-pass algorithmic bytes of the whole batch"}
+                        "bytes_model": "algorithmic bytes of the executed schedule for the whole batch"}
     if per_size:
         line["per_size"] = per_size
     bp.close()

@@ -11,11 +11,11 @@ struct CfBatchSeg {
   float scale;      // final scale (x^-1 pass)
   int n;            // cube edge
   int kind;         // 0 x fwd, 1 y fwd, 2 z*H*z, 3 y inv, 4 x inv
-  int first_item;   // queue index of the segment's first tile
+  int first_item;   // queue index of the segment's first work item
   int ntiles;       // tiles of the pass
+  int per_item;     // tiles per work item (the last item of a segment may hold fewer)
   int dep;          // index of the segment that must complete first, -1 if none
   int dep_need;     // tile count of that segment
-  int pad;
 };
 static_assert(sizeof(CfBatchSeg) == 64, "CfBatchSeg must be 64 bytes");
 

@@ -989,6 +989,8 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
     // segments: pass-major, instance-minor
     std::vector<CfBatchSeg> segs;
     int item = 0;
+    int per_item = 8;  // tiles per work item
+    if (const char* e = getenv("CONVFPGA_BATCH_TILES_PER_ITEM")) per_item = std::max(1, atoi(e));
     for (int s = 0; s < kBatchKinds; ++s)
       for (unsigned i = 0; i < count; ++i) {
         const unsigned n = inst[i].h->n;
@@ -1003,9 +1005,10 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
         sg.kind = s;
         sg.first_item = item;
         sg.ntiles = row ? cfgeo::row_tiles((int)n) : cfgeo::col_tiles((int)n);
+        sg.per_item = per_item;
         sg.dep = s == 0 ? -1 : (int)((s - 1) * count + i);
         sg.dep_need = s == 0 ? 0 : segs[(size_t)(s - 1) * count + i].ntiles;
-        item += sg.ntiles;
+        item += (sg.ntiles + per_item - 1) / per_item;
         segs.push_back(sg);
       }
     unsigned long long spin_limit = 1ull << 21;

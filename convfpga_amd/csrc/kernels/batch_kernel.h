@@ -30,7 +30,7 @@ namespace cfk {
 
 enum BatchPassKind : int { BK_ROW_FWD = 0, BK_COLY_FWD = 1, BK_COLZ_CONV = 2, BK_COLY_INV = 3, BK_ROW_INV = 4 };
 
-// ctr[0]: queue head, ctr[1]: error flag, ctr[2 + s]: completed tiles of segment s
+// ctr[0]: queue head (work items), ctr[1]: error flag, ctr[2 + s]: completed tiles of segment s
 // (kept out of line: inlining every size into one switch makes the register
 // allocator take far more VGPRs than any single size needs)
 template <int N>
@@ -89,10 +89,12 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
   // the bottom of the body lets the compiler peel an exec-masked inner loop for
   // the other lanes, in which they pass the top barrier again without thread 0
   // and spin on a stale item.
-  int prev = -1;  // segment of the tile just finished (uniform)
+  int prev = -1;  // segment of the item just finished (uniform)
+  int prev_tiles = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      if (prev >= 0) __hip_atomic_fetch_add(&ctr[2 + prev], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev >= 0)
+        __hip_atomic_fetch_add(&ctr[2 + prev], (unsigned)prev_tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       const int item = (int)atomicAdd(&ctr[0], 1u);
       int seg = -1;
       if (item < nitems) {  // segments are sorted by first_item: binary search
@@ -129,21 +131,28 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
     const CfBatchSeg sg = segs[seg];
     for (int i = threadIdx.x; i < sg.n; i += 256) tw_s[i] = ((const float2*)sg.tw)[i];
     __syncthreads();
-    const int tile = item - sg.first_item;
+    // a work item is a run of up to per_item consecutive tiles of the segment:
+    // one release per item instead of per tile (each device-scope release
+    // writes back the XCD's L2)
+    const int t0 = (item - sg.first_item) * sg.per_item;
+    const int t1 = min(t0 + sg.per_item, sg.ntiles);
     const float2* in = (const float2*)sg.in;
     float2* out = (float2*)sg.out;
     const float2* H = (const float2*)sg.H;
-    switch (sg.n) {
-      case 16: batch_tile<16>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-      case 32: batch_tile<32>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-      case 64: batch_tile<64>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-      case 128: batch_tile<128>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-      case 256: batch_tile<256>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-      default: break;
+    for (int tile = t0; tile < t1; ++tile) {
+      switch (sg.n) {
+        case 16: batch_tile<16>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
+        case 32: batch_tile<32>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
+        case 64: batch_tile<64>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
+        case 128: batch_tile<128>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
+        case 256: batch_tile<256>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
+        default: break;
+      }
     }
     __threadfence();  // this thread's stores are visible device-wide ...
-    __syncthreads();  // ... for every thread of the tile (thread 0 publishes next turn); LDS free again
+    __syncthreads();  // ... for every thread of the item (thread 0 publishes next turn); LDS free again
     prev = seg;
+    prev_tiles = t1 - t0;
   }
 }
 
