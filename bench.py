@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--plan", type=int, default=0, help="force a schedule (CF_PLAN_*); 0 = autotuned")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--host-calls", action="store_true",
+                    help="also time the drop-in calls on host buffers (PCIe included; not the metric)")
     return ap.parse_args()
 
 
@@ -114,11 +116,16 @@ def base_line(a, metric, value, unit, ms_per_step, config):
             "dtype": "fp32", "data": "synthetic (seeded U[-1,1) re/im generated on the GPU)", "config": config}
 
 
-KERNEL_OF_PASS = {  # kernel name prefix of each pass name, for the traffic lookup
-    "xy1_fwd": "cf_xy1_fwd_256", "xy1_inv": "cf_xy1_inv_256", "y2z_fwd_mul_inv": "cf_mid_256",
-    "x_fwd": "cf_row_fwd", "x_inv": "cf_row_inv", "y_fwd": "cf_coly_fwd", "y_inv": "cf_coly_inv",
-    "z_fwd_mul_z_inv": "cf_colz_conv", "xy_fwd": "cf_plane_fwd", "xy_inv": "cf_plane_inv",
+KERNEL_OF_PASS = {  # kernel of each pass name (N = edge; G, GM = tile-group suffixes of the four-step plans)
+    "xy1_fwd": "cf_xy1_fwd_{N}{G}", "xy1_inv": "cf_xy1_inv_{N}{G}", "y2z_fwd_mul_inv": "cf_mid_{N}{GM}",
+    "x_fwd": "cf_row_fwd_{N}", "x_inv": "cf_row_inv_{N}", "y_fwd": "cf_coly_fwd_{N}", "y_inv": "cf_coly_inv_{N}",
+    "z_fwd_mul_z_inv": "cf_colz_conv_{N}", "xy_fwd": "cf_plane_fwd_{N}", "xy_inv": "cf_plane_inv_{N}",
 }
+
+
+def kernel_of_pass(name, n, plan_name):
+    gm = "_" + plan_name.split("_")[1] if plan_name.startswith("split3_") else ""
+    return KERNEL_OF_PASS.get(name, "").format(N=n, G=gm.split("m")[0], GM=gm)
 
 
 def run_single(a, torch, C, api):
@@ -159,19 +166,17 @@ def run_single(a, torch, C, api):
         p["GBps"] = p["bytes"] / (p["ms"] * 1e-3) / 1e9
         p["frac_of_peak"] = p["GBps"] / HBM_PEAK_GBS
     dom = max(passes, key=lambda p: p["ms"])
-    kprefix = KERNEL_OF_PASS.get(dom["name"], "")
-    traffic = None
+    kname = kernel_of_pass(dom["name"], n, api.PLAN_NAMES[plan])
+    traffic = None  # HBM bytes per launch from the committed rocprof counter runs (FETCH_SIZE, WRITE_SIZE)
     tfile = os.path.join(ROOT, "profiles", "traffic_256.json")
-    if n == 256 and kprefix and os.path.exists(tfile):
-        for k, v in json.load(open(tfile)).items():
-            if v.get("plan") == api.PLAN_NAMES[plan] and k.startswith(kprefix):
-                traffic = v["traffic_bytes"]
+    if n == 256 and os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(kname, {}).get("traffic_bytes")
     line = base_line(a, "3D convs/sec + %% HBM roofline at %d^3 c2c" % n, a.steps / wall, "conv/s",
                      1e3 * wall / a.steps,
                      {"workload": "conv3d_%d^3_c2c_fp32_batch1_precomputed_filter_spectrum" % n, "N": n, "batch": 1,
                       "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
                       "algorithmic_bytes_per_step": alg_bytes})
-    line["roofline"] = {"bound": "hbm", "kernel": kprefix, "pass": dom["name"], "achieved": dom["GBps"],
+    line["roofline"] = {"bound": "hbm", "kernel": kname, "pass": dom["name"], "achieved": dom["GBps"],
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic}
     line["conv_roofline_frac"] = alg_bytes * a.steps / wall / 1e9 / HBM_PEAK_GBS
     # the same rate priced with the 5-pass model: 11 volume transfers per convolution
@@ -181,7 +186,32 @@ def run_single(a, torch, C, api):
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
+    if a.host_calls:
+        line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
     return line, (n, host_out, seed_sig, seed_filt)
+
+
+def dropin_host_leg(C, n, sig, filt, reps=3):
+    """The drop-in calls on pageable host buffers (what the reference's callers
+    pass): every phase including the PCIe copies and the filter transform, per
+    call.  Not the metric -- reported beside it."""
+    import numpy as np
+    hs, hf = sig.cpu().numpy(), filt.cpu().numpy()
+    ho = np.empty_like(hs)
+    res = {}
+    for name, fn in (("fpgaf_conv3D", C.fpgaf_conv3D), ("fpgaf_conv3D_svm", C.fpgaf_conv3D_svm)):
+        walls, fields = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            t = fn(n, hs, hf, ho)
+            walls.append(time.perf_counter() - t0)
+            if not t.valid:
+                raise SystemExit("%s returned valid=false: %s" % (name, C.api.last_error()))
+            fields.append(t.as_dict())
+        k = min(range(reps), key=lambda i: walls[i])
+        res[name] = {"wall_ms_best": round(1e3 * walls[k], 3), "conv_per_s": 1.0 / walls[k],
+                     "fpga_t_ms": {f: round(v, 4) for f, v in fields[k].items() if f != "valid"}}
+    return res
 
 
 def run_batch(a, torch, C, api):
@@ -294,7 +324,7 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     torch.zeros(1, device="cuda")
-    rc = C.fpga_initialize("gfx950", C.CODE_OBJECT, False)
+    rc = C.fpga_initialize("gfx950", C.CODE_OBJECT, True)
     if rc != 0:
         raise SystemExit("fpga_initialize -> %d" % rc)
     if a.workload in SINGLE:

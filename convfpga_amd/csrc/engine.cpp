@@ -61,14 +61,15 @@ struct SizeFns {
   hipFunction_t row_fwd = nullptr, row_inv = nullptr, coly_fwd = nullptr, coly_inv = nullptr;
   hipFunction_t colz_fwd = nullptr, colz_inv = nullptr, colz_conv = nullptr;
   hipFunction_t plane_fwd = nullptr, plane_inv = nullptr;                 // N in {32, 64, 128}
-  hipFunction_t xy1_fwd[4] = {}, xy1_inv[4] = {}, mid[4] = {}, hlay[4] = {};  // N = 256
+  hipFunction_t xy1_fwd[cfgeo::kSplitVariants] = {}, xy1_inv[cfgeo::kSplitVariants] = {};  // N = 256
+  hipFunction_t mid[cfgeo::kSplitVariants] = {}, hlay[cfgeo::kSplitVariants] = {};
   void* tw = nullptr;  // N float2 twiddles W_N^j
   int forced = CF_PLAN_AUTO;  // cf_set_plan
   int tuned = CF_PLAN_AUTO;   // autotuner's choice, CF_PLAN_AUTO until it has run
   float tune_ms[CF_NUM_PLANS] = {};
 };
 
-bool is_split(int plan) { return plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G44; }
+bool is_split(int plan) { return plan >= CF_PLAN_SPLIT3_G22 && plan < CF_PLAN_SPLIT3_G22 + cfgeo::kSplitVariants; }
 int split_variant(int plan) { return plan - CF_PLAN_SPLIT3_G22; }
 
 bool plan_ok(unsigned n, int plan) {
@@ -240,15 +241,20 @@ int size_res(unsigned n, SizeFns** out) {
     CFCHK(get_fn(&s.plane_inv, name));
   }
   if (n == 256) {
-    for (int vi = 0; vi < 4; ++vi) {
-      const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi);
+    for (int vi = 0; vi < cfgeo::kSplitVariants; ++vi) {
+      const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
+      char sfx[16];
+      if (gm == gb)
+        snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
+      else
+        snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
       snprintf(name, sizeof name, "cf_xy1_fwd_256_g%d%d", ga, gb);
       CFCHK(get_fn(&s.xy1_fwd[vi], name));
       snprintf(name, sizeof name, "cf_xy1_inv_256_g%d%d", ga, gb);
       CFCHK(get_fn(&s.xy1_inv[vi], name));
-      snprintf(name, sizeof name, "cf_mid_256_g%d%d", ga, gb);
+      snprintf(name, sizeof name, "cf_mid_256_%s", sfx);
       CFCHK(get_fn(&s.mid[vi], name));
-      snprintf(name, sizeof name, "cf_hlayout_256_g%d%d", ga, gb);
+      snprintf(name, sizeof name, "cf_hlayout_256_%s", sfx);
       CFCHK(get_fn(&s.hlay[vi], name));
     }
   }
@@ -280,8 +286,10 @@ struct Pass {
 int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const void* tw) {
   const size_t ce = cube_elems(n);
   // tiled passes cover the whole batch with one 1-D grid (one work-group per
-  // tile, tile indices in int); the others use gridDim.y for the cube
-  const unsigned max_cubes = p.tiled ? std::max(1u, (unsigned)(0x7fffffff / std::max(1u, p.tiles))) : 65535u;
+  // tile, tile indices in int, grid size in work-items below 2^31); the
+  // others use gridDim.y for the cube
+  const unsigned long long per_cube = (unsigned long long)std::max(1u, p.tiles) * p.threads;
+  const unsigned max_cubes = p.tiled ? (unsigned)std::max(1ull, 0x7fffffffull / per_cube) : 65535u;
   for (unsigned b0 = 0; b0 < nb; b0 += max_cubes) {
     const unsigned cnt = std::min(max_cubes, nb - b0);
     const void* in = (const char*)p.in + b0 * ce * 8;
@@ -292,11 +300,8 @@ int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const voi
     const void* twp = tw;
     int ntotal = (int)(p.tiles * cnt);
     void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc, (void*)&ntotal};
-    if (p.tiled) {
-      HIPCHK(hipModuleLaunchKernel(p.f, (unsigned)ntotal, 1, 1, p.threads, 1, 1, 0, s, args, nullptr));
-    } else {
-      HIPCHK(hipModuleLaunchKernel(p.f, p.tiles, cnt, 1, p.threads, 1, 1, 0, s, args, nullptr));
-    }
+    const unsigned gx = p.tiled ? (unsigned)ntotal : p.tiles, gy = p.tiled ? 1u : cnt;
+    HIPCHK(hipModuleLaunchKernel(p.f, gx, gy, 1, p.threads, 1, 1, 0, s, args, nullptr));
   }
   return CF_OK;
 }
@@ -419,13 +424,25 @@ int acquire_work(size_t bytes, hipStream_t s, void** out) {
     if (e != hipSuccess) return fail(CF_ERR_NOMEM, "hipMalloc(%zu) for the workspace: %s", bytes, hipGetErrorString(e));
     g.d_work_bytes = bytes;
   }
-  if (g.work_used && g.work_stream != s) HIPCHK(hipStreamWaitEvent(s, g.work_ev, 0));
+  if (g.work_used && g.work_stream != s) {
+    // Everything submitted so far to the previous user's stream includes its
+    // use of the workspace, so an event recorded there now orders s after it.
+    // (Recording lazily, only on a change of stream, keeps event packets --
+    // each a gap of several microseconds on the GPU -- out of the
+    // single-stream sequence.)  A stream destroyed meanwhile has finished its
+    // work; should the record fail, wait for the whole device instead.
+    if (hipEventRecord(g.work_ev, g.work_stream) == hipSuccess) {
+      HIPCHK(hipStreamWaitEvent(s, g.work_ev, 0));
+    } else {
+      (void)hipGetLastError();
+      HIPCHK(hipDeviceSynchronize());
+    }
+  }
   *out = g.d_work;
   return CF_OK;
 }
 
 int release_work(hipStream_t s) {
-  HIPCHK(hipEventRecord(g.work_ev, s));
   g.work_stream = s;
   g.work_used = true;
   return CF_OK;
@@ -433,7 +450,7 @@ int release_work(hipStream_t s) {
 
 int ensure_events() {
   for (auto& e : g.ev)
-    if (!e) HIPCHK(hipEventCreate(&e));
+    if (!e) HIPCHK(hipEventCreate(&e));  // system-scope fence: host reads (SVM output) follow these
   return CF_OK;
 }
 
@@ -840,7 +857,9 @@ extern "C" int cf_profile_begin(unsigned max_calls) {
   CFCHK(require_ready());
   for (auto e : g.prof_ev) hipEventDestroy(e);
   g.prof_ev.assign((size_t)max_calls * (CF_MAX_PASSES + 1), nullptr);
-  for (auto& e : g.prof_ev) HIPCHK(hipEventCreate(&e));
+  // timing-only events: no system-scope fence (cache writeback) when they
+  // complete, which would otherwise stall the next pass for microseconds
+  for (auto& e : g.prof_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   g.prof_max = max_calls;
   g.prof_calls = 0;
   return CF_OK;

@@ -32,12 +32,18 @@ constexpr bool plane_supported(int n) { return n == 32 || n == 64 || n == 128; }
 constexpr int plane_threads(int n) { return n * threads_per_line(n); }
 constexpr int plane_tiles(int n) { return n; }
 
-// 3-pass four-step schedule (N = 256): tile groups GA (pass A/C) and GB (middle)
-constexpr int split_ga(int variant) { return variant == 0 ? 2 : variant == 1 ? 2 : variant == 2 ? 4 : 4; }
-constexpr int split_gb(int variant) { return variant == 0 ? 2 : variant == 1 ? 4 : variant == 2 ? 2 : 4; }
-constexpr int split_a_threads(int variant) { return 256 * split_ga(variant); }
-constexpr int split_a_tiles(int variant) { return 256 * (16 / split_ga(variant)); }
-constexpr int split_b_threads(int variant) { return 256 * split_gb(variant); }
-constexpr int split_b_tiles(int variant) { return 16 * (256 / split_gb(variant)); }
+// 3-pass four-step schedule (N = 256).  Variant v: GA = y2 values per
+// outer-pass tile, GB = x group of the intermediate layout (runs of GA*GB
+// points), GM = x values per middle-pass tile (GM divides GB).
+constexpr int kSplitVariants = 9;
+constexpr int split_ga(int v) { return v == 2 || v == 3 ? 4 : 2; }
+constexpr int split_gb(int v) { return v == 0 || v == 2 ? 2 : v <= 5 ? 4 : 8; }
+constexpr int split_gm(int v) {
+  return v == 0 || v == 2 || v == 4 || v == 7 ? 2 : v == 1 || v == 3 || v == 6 ? 4 : 1;
+}
+constexpr int split_a_threads(int v) { return 256 * split_ga(v); }
+constexpr int split_a_tiles(int v) { return 256 * (16 / split_ga(v)); }
+constexpr int split_b_threads(int v) { return 256 * split_gm(v); }
+constexpr int split_b_tiles(int v) { return 16 * (256 / split_gm(v)); }
 
 }  // namespace cfgeo

@@ -75,32 +75,48 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
                   cfk::PlaneCfg<32>::THREADS == cfgeo::plane_threads(32),
               "plane geometry");
 
-#define CF_SPLIT(GA, GB)                                                                                        \
+#define CF_SPLIT_AC(GA, GB)                                                                                     \
   extern "C" __global__ __launch_bounds__(256 * GA) void cf_xy1_fwd_256_g##GA##GB(CF_ARGS) {                    \
     cfk::xy1_kernel<GA, GB, false>(in, out, tw, scale, ntotal);                                                 \
   }                                                                                                             \
   extern "C" __global__ __launch_bounds__(256 * GA) void cf_xy1_inv_256_g##GA##GB(CF_ARGS) {                    \
     cfk::xy1_kernel<GA, GB, true>(in, out, tw, scale, ntotal);                                                  \
-  }                                                                                                             \
-  extern "C" __global__ __launch_bounds__(256 * GB) void cf_mid_256_g##GA##GB(CF_ARGS) {                        \
-    cfk::mid_kernel<GA, GB>(in, out, H, tw, hstride, ntotal);                                                   \
-  }                                                                                                             \
-  extern "C" __global__ __launch_bounds__(256) void cf_hlayout_256_g##GA##GB(CF_ARGS) {                         \
-    cfk::h_layout_body<GA, GB>(in, out);                                                                        \
   }
-CF_SPLIT(2, 2)
-CF_SPLIT(2, 4)
-CF_SPLIT(4, 2)
-CF_SPLIT(4, 4)
+#define CF_SPLIT_B(GA, GB, GM, SUFFIX)                                                                          \
+  extern "C" __global__ __launch_bounds__(256 * GM) void cf_mid_256_##SUFFIX(CF_ARGS) {                         \
+    cfk::mid_kernel<GA, GB, GM>(in, out, H, tw, hstride, ntotal);                                               \
+  }                                                                                                             \
+  extern "C" __global__ __launch_bounds__(256) void cf_hlayout_256_##SUFFIX(CF_ARGS) {                          \
+    cfk::h_layout_body<GM>(in, out);                                                                            \
+  }
+// variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>
+CF_SPLIT_AC(2, 2)
+CF_SPLIT_AC(2, 4)
+CF_SPLIT_AC(4, 2)
+CF_SPLIT_AC(4, 4)
+CF_SPLIT_AC(2, 8)
+CF_SPLIT_B(2, 2, 2, g22)
+CF_SPLIT_B(2, 4, 4, g24)
+CF_SPLIT_B(4, 2, 2, g42)
+CF_SPLIT_B(4, 4, 4, g44)
+CF_SPLIT_B(2, 4, 2, g24m2)
+CF_SPLIT_B(2, 4, 1, g24m1)
+CF_SPLIT_B(2, 8, 4, g28m4)
+CF_SPLIT_B(2, 8, 2, g28m2)
+CF_SPLIT_B(2, 8, 1, g28m1)
 template <int V>
 struct SplitGeoCheck {
   using K = cfk::Split256<cfgeo::split_ga(V), cfgeo::split_gb(V)>;
+  using M = cfk::MidGeo<cfgeo::split_gb(V), cfgeo::split_gm(V)>;
   static_assert(K::A_THREADS == cfgeo::split_a_threads(V) && K::A_TILES == cfgeo::split_a_tiles(V) &&
-                    K::B_THREADS == cfgeo::split_b_threads(V) && K::B_TILES == cfgeo::split_b_tiles(V),
+                    M::THREADS == cfgeo::split_b_threads(V) && M::TILES == cfgeo::split_b_tiles(V),
                 "split geometry");
   static constexpr bool ok = true;
 };
-static_assert(SplitGeoCheck<0>::ok && SplitGeoCheck<1>::ok && SplitGeoCheck<2>::ok && SplitGeoCheck<3>::ok, "");
+static_assert(SplitGeoCheck<0>::ok && SplitGeoCheck<1>::ok && SplitGeoCheck<2>::ok && SplitGeoCheck<3>::ok &&
+                  SplitGeoCheck<4>::ok && SplitGeoCheck<5>::ok && SplitGeoCheck<6>::ok && SplitGeoCheck<7>::ok &&
+                  SplitGeoCheck<8>::ok && cfgeo::kSplitVariants == 9,
+              "");
 
 // ---- persistent batch of mixed sizes (batch_kernel.h) -----------------------------
 extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(

@@ -15,10 +15,12 @@
 //   xy1_inv : twiddle W256^(-y2 k1), inverse 16-point DFT over k1, inverse x
 //             FFT, final scale, natural order out
 // The intermediate between the passes (in the workspace) is indexed by
-// (x, k1, y2, z) and laid out [k1][x/GB][z][y2][x%GB], so that a mid_conv tile
-// (k1, x/GB) is one contiguous block of 4096*GB points and xy1_* move runs of
-// GA*GB contiguous points.  The filter spectrum for mid_conv is stored
-// [k1][x/GB][k2][kz][x%GB] so that it is read with unit stride as well.
+// (x, k1, y2, z) and laid out [k1][x/GB][z][y2][x%GB]: xy1_* move runs of
+// GA*GB contiguous points.  A mid_conv tile is (k1, GM adjacent x) with GM
+// dividing GB: for GM = GB one contiguous block of 4096*GB points, for GM < GB
+// runs of GM points whose sibling tiles (same k1, same x/GB) run on the same
+// XCD.  The filter spectrum for mid_conv is stored [k1][x/GM][k2][kz][x%GM] so
+// that each tile reads its own contiguous block.
 #pragma once
 #include "fft_core.h"
 
@@ -124,14 +126,6 @@ struct Split256 {
   // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
   static constexpr int A_COL_PLANE = 256 * 17 + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
-  static constexpr int B_THREADS = 256 * GB;       // 16*GB z-lines of 16 threads (launch bound of mid)
-  static constexpr int B_TILES = 16 * NXH;         // (k1, x block) per cube
-  static constexpr int B_LINES = 16 * GB;          // z-lines per tile: (y2, x%GB)
-  // row pitch 17*GB (= GB mod 32): the (kz, x%GB) reads of the y2 transpose
-  // step 2*GB banks per kz, so a half-wave touches every bank pair once
-  static constexpr int B_PITCH = 17 * GB;
-  static constexpr int B_LDS = 256 * B_PITCH;
-  static constexpr int TILE = 4096 * GB;           // points per mid tile
   // intermediate index of (x, k1, y2, z)
   __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
     return (((k1 * NXH + x / GB) * 256 + z) * 16 + y2) * GB + (x % GB);
@@ -249,11 +243,28 @@ __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const 
   k.process(tile, a);
 }
 
-// Middle pass: one contiguous tile (k1, x block) = [z][y2][x%GB].
-template <int GA, int GB>
+// Middle pass geometry: tile = (k1, GM adjacent x) of a layout with x-group GB.
+template <int GB, int GM>
+struct MidGeo {
+  static_assert(GB % GM == 0, "the mid tile width must divide the layout's x group");
+  static constexpr int NXM = 256 / GM;          // mid tiles per k1
+  static constexpr int THREADS = 256 * GM;      // 16*GM z-lines of 16 threads (launch bound of mid)
+  static constexpr int TILES = 16 * NXM;        // (k1, x block) per cube
+  static constexpr int LINES = 16 * GM;         // z-lines per tile: (y2, x%GM)
+  // row pitch 17*GM (= GM mod 32): the (kz, x%GM) reads of the y2 transpose
+  // step 2*GM banks per kz, so a half-wave touches every bank pair once
+  static constexpr int PITCH = 17 * GM;
+  static constexpr int LDS = 256 * PITCH;
+  static constexpr int HTILE = 4096 * GM;       // spectrum points per tile
+  static constexpr int SIBLINGS = GB / GM;      // tiles sharing the layout's x group
+};
+
+// Middle pass on tile (k1, x0 .. x0+GM-1).
+template <int GA, int GB, int GM>
 struct MidPass {
   using K = Split256<GA, GB>;
-  static constexpr int N = 256, L = K::B_LINES;
+  using M = MidGeo<GB, GM>;
+  static constexpr int N = 256, L = M::LINES;
   struct Regs {
     float2 v[16];
   };
@@ -263,19 +274,23 @@ struct MidPass {
   const float2* tw_s;
   float2* lds;
   long long hstride;
-  int l, zt;  // z-line identity: line l = y2*GB + x%GB, thread zt holds z = zt + 16*m
-  int qx, qz; // (kz, x%GB) identity for the y2 transform
+  int l, zt;  // z-line identity: line l = y2*GM + x%GM, thread zt holds z = zt + 16*m
+  int qx, qz; // (kz, x%GM) identity for the y2 transform
 
-  __device__ __forceinline__ size_t tile_off(int tile) const {
-    return (size_t)(tile / K::B_TILES) * N * N * N + (size_t)(tile % K::B_TILES) * K::TILE;
+  // offset of this thread's point z = zt (m = 0); point m is 16*16*GB*m further
+  __device__ __forceinline__ size_t thread_off(int tile) const {
+    const int tt = tile % M::TILES;
+    const int x0 = (tt % M::NXM) * GM;
+    const size_t cube = (size_t)(tile / M::TILES) * N * N * N;
+    return cube + K::mid_index(x0 + l % GM, tt / M::NXM, l / GM, zt);
   }
   __device__ __forceinline__ void load(int tile, Regs& r) const {
-    const float2* src = in + tile_off(tile) + threadIdx.x;
+    const float2* src = in + thread_off(tile);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) r.v[m] = src[K::B_THREADS * m];
+    for (int m = 0; m < 16; ++m) r.v[m] = src[256 * GB * m];
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
-    constexpr int PITCH = K::B_PITCH;
+    constexpr int PITCH = M::PITCH;
     const PitchLds<PITCH> view{lds + l};
     fft_line<N, false>(r.v, zt, view, tw_s);  // r.v[m] = Z(kz = zt + 16 m) of line l
 #pragma unroll
@@ -284,55 +299,60 @@ struct MidPass {
     float2 u[16];
     float2* rowp = lds + qz * PITCH + qx;
 #pragma unroll
-    for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GB];
-    const int cube = tile / K::B_TILES;
-    const float2* h = H + (size_t)cube * hstride + (size_t)(tile % K::B_TILES) * K::TILE + qz * GB + qx;
+    for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GM];
+    const int cube = tile / M::TILES;
+    const float2* h = H + (size_t)cube * hstride + (size_t)(tile % M::TILES) * M::HTILE + qz * GM + qx;
     float2 hv[16];
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GB];
+    for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GM];
     DFT<16, false>::run(u);  // u[k2]: full 3D spectrum at (x, k1 + 16 k2, qz)
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) u[k2] = cmul(u[k2], hv[k2]);
     DFT<16, true>::run(u);  // u[y2]
     __syncthreads();  // every thread has read its row before it is overwritten
 #pragma unroll
-    for (int y2 = 0; y2 < 16; ++y2) rowp[y2 * GB] = u[y2];
+    for (int y2 = 0; y2 < 16; ++y2) rowp[y2 * GM] = u[y2];
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + 16 * m) * PITCH + l];
     __syncthreads();
     fft_line<N, true>(r.v, zt, view, tw_s);
-    float2* dst = out + tile_off(tile) + threadIdx.x;
+    float2* dst = out + thread_off(tile);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) dst[K::B_THREADS * m] = r.v[m];
+    for (int m = 0; m < 16; ++m) dst[256 * GB * m] = r.v[m];
   }
 };
 
-template <int GA, int GB>
+template <int GA, int GB, int GM>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
-  using K = Split256<GA, GB>;
-  __shared__ float2 lds[K::B_LDS];
+  using M = MidGeo<GB, GM>;
+  static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
+  __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[256];
-  stage_twiddles<256, K::B_THREADS>(tw_s, tw);
+  stage_twiddles<256, M::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
-  const MidPass<GA, GB> k{in, out, H, tw_s, lds, hstride, x % K::B_LINES, x / K::B_LINES, x % GB, x / GB};
-  run_tiles(k, ntotal);
+  const MidPass<GA, GB, GM> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x % GM, x / GM};
+  // siblings (consecutive tiles of one x group) on one XCD, back to back
+  const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
+  if (tile >= ntotal) return;
+  typename MidPass<GA, GB, GM>::Regs a;
+  k.load(tile, a);
+  k.process(tile, a);
 }
 
-// Spectrum re-layout for mid_body: natural FFT3 order -> [k1][x/GB][k2][kz][x%GB].
-template <int GA, int GB>
+// Spectrum re-layout for the middle pass: natural FFT3 order -> [k1][x/GM][k2][kz][x%GM].
+template <int GM>
 __device__ __forceinline__ void h_layout_body(const float2* __restrict__ in, float2* __restrict__ out) {
-  using K = Split256<GA, GB>;
-  constexpr int N = 256;
+  constexpr int N = 256, NXM = N / GM;
   const int i = blockIdx.x * 256 + threadIdx.x;  // output index within the cube
   const size_t cube = (size_t)blockIdx.y * N * N * N;
-  const int xl = i % GB;
-  const int kz = (i / GB) % 256;
-  const int k2 = (i / (GB * 256)) % 16;
-  const int xh = (i / (GB * 256 * 16)) % K::NXH;
-  const int k1 = i / (GB * 256 * 16 * K::NXH);
-  const int x = xh * GB + xl, ky = k1 + 16 * k2;
+  const int xl = i % GM;
+  const int kz = (i / GM) % 256;
+  const int k2 = (i / (GM * 256)) % 16;
+  const int xh = (i / (GM * 256 * 16)) % NXM;
+  const int k1 = i / (GM * 256 * 16 * NXM);
+  const int x = xh * GM + xl, ky = k1 + 16 * k2;
   out[cube + i] = in[cube + ((size_t)kz * N + ky) * N + x];
 }
 

@@ -41,7 +41,14 @@ extern "C" {
 #define CF_PLAN_SPLIT3_G24 4 /*   ... tile groups 2x4                                          */
 #define CF_PLAN_SPLIT3_G42 5 /*   ... tile groups 4x2                                          */
 #define CF_PLAN_SPLIT3_G44 6 /*   ... tile groups 4x4                                          */
-#define CF_NUM_PLANS 7
+/* the same with middle-pass tiles narrower than the intermediate's x group:
+ * G<GA><GB>M<GM> = outer tiles of GA y2 values, runs of GA*GB points, middle tiles of GM x */
+#define CF_PLAN_SPLIT3_G24M2 7
+#define CF_PLAN_SPLIT3_G24M1 8
+#define CF_PLAN_SPLIT3_G28M4 9
+#define CF_PLAN_SPLIT3_G28M2 10
+#define CF_PLAN_SPLIT3_G28M1 11
+#define CF_NUM_PLANS 12
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
@@ -98,7 +105,8 @@ int cf_conv3d_device_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsi
 /* Event profiling of the next cf_conv3d_device calls without a host sync per
  * call: cf_profile_begin arms it for up to max_calls calls, cf_profile_end
  * waits for them and returns each pass's mean device time over the recorded
- * calls (times->ms) and the number of calls recorded. */
+ * calls (times->ms, from events recorded on the launch stream between the
+ * passes) and the number of calls recorded. */
 int cf_profile_begin(unsigned max_calls);
 int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
 

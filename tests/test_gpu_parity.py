@@ -210,8 +210,8 @@ def test_device_fill_matches_host_generator(gpu, oracle):
     assert np.array_equal(_np(x).view(np.uint32), want.view(np.uint32))
 
 
-PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1), (256, 3), (256, 4), (256, 5), (256, 6),
-              (512, 1)]
+PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
+    [(256, p) for p in range(3, 12)] + [(512, 1)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
