@@ -63,6 +63,7 @@ struct SizeFns {
   hipFunction_t plane_fwd = nullptr, plane_inv = nullptr;                 // N in {32, 64, 128}
   hipFunction_t xy1_fwd[cfgeo::kSplitVariants] = {}, xy1_inv[cfgeo::kSplitVariants] = {};  // N = 256
   hipFunction_t mid[cfgeo::kSplitVariants] = {}, hlay[cfgeo::kSplitVariants] = {};
+  hipFunction_t mx_xy1_fwd = nullptr, mx_xy1_inv = nullptr, mx_mid = nullptr;  // N = 256, split-FP16 MFMA
   void* tw = nullptr;  // N float2 twiddles W_N^j
   int forced = CF_PLAN_AUTO;  // cf_set_plan
   int tuned = CF_PLAN_AUTO;   // autotuner's choice, CF_PLAN_AUTO until it has run
@@ -87,7 +88,7 @@ struct Engine {
   int device = 0;
   hipModule_t mod = nullptr;
   hipStream_t stream = nullptr;
-  hipFunction_t twgen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr;
+  hipFunction_t twgen = nullptr, dft16gen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr;
   hipFunction_t batch_run = nullptr, batch_reset = nullptr;
   hipStream_t capture = nullptr;  // private stream for graph capture
   std::map<unsigned, SizeFns> sizes;
@@ -166,7 +167,7 @@ void release_all() {
   g.stream = nullptr;
   if (g.mod) hipModuleUnload(g.mod);
   g.mod = nullptr;
-  g.twgen = g.errp = g.errf = g.fillu = g.batch_run = g.batch_reset = nullptr;
+  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.batch_run = g.batch_reset = nullptr;
   if (g.capture) hipStreamDestroy(g.capture);
   g.capture = nullptr;
   g.ready = false;
@@ -257,13 +258,24 @@ int size_res(unsigned n, SizeFns** out) {
       snprintf(name, sizeof name, "cf_hlayout_256_%s", sfx);
       CFCHK(get_fn(&s.hlay[vi], name));
     }
+    CFCHK(get_fn(&s.mx_xy1_fwd, "cf_xy1_fwd_256_s24"));
+    CFCHK(get_fn(&s.mx_xy1_inv, "cf_xy1_inv_256_s24"));
+    CFCHK(get_fn(&s.mx_mid, "cf_mid_256_s24m2"));
   }
   read_tune_file(n, &s);
-  HIPCHK(hipMalloc(&s.tw, (size_t)n * 8));
+  // twiddles W_N^j; for N = 256 followed by the DFT16 operand table of the
+  // split-FP16 kernels (2 x 64 lanes x 4 x 16 bytes)
+  const size_t tw_bytes = (size_t)n * 8, table_bytes = n == 256 ? 8192 : 0;
+  HIPCHK(hipMalloc(&s.tw, tw_bytes + table_bytes));
   void* tw = s.tw;
   int nn = (int)n;
   void* args[] = {&tw, &nn};
   HIPCHK(hipModuleLaunchKernel(g.twgen, (n + 255) / 256, 1, 1, 256, 1, 1, 0, g.stream, args, nullptr));
+  if (table_bytes) {
+    void* table = (char*)s.tw + tw_bytes;
+    void* targs[] = {&table};
+    HIPCHK(hipModuleLaunchKernel(g.dft16gen, 1, 1, 1, 128, 1, 1, 0, g.stream, targs, nullptr));
+  }
   HIPCHK(hipStreamSynchronize(g.stream));
   *out = &(g.sizes[n] = s);
   return CF_OK;
@@ -330,10 +342,19 @@ void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void*
 // SPLIT3: sig -> work (pass A), work in place (middle), work -> out (pass C).
 // Returns the index of the first pass after the product (the fpga_t split).
 size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const void* sig, void* out, void* work,
-                   const void* H, long long hstride, float scale, std::vector<Pass>& v) {
+                   const void* H, long long hstride, float scale, std::vector<Pass>& v,
+                   int precision = CF_PREC_FP32) {
   const double V = (double)cube_bytes(n);
   const double io = 2.0 * V * nb;
   const double hbytes = hstride ? V * nb : V;
+  if (precision == CF_PREC_SPLIT_FP16) {  // four-step plan with middle tiles of 2 x (checked by the caller)
+    const unsigned at = 512, aT = 256 * 8, bt = 512, bT = 16 * 128;
+    v.push_back({f.mx_xy1_fwd, aT, at, sig, work, nullptr, 0, 1.0f, "xy1_fwd_mfma", io});
+    v.push_back({f.mx_mid, bT, bt, work, work, H, hstride, 1.0f, "y2z_mfma_mul_inv", io + hbytes});
+    const size_t split = v.size();
+    v.push_back({f.mx_xy1_inv, aT, at, work, out, nullptr, 0, scale, "xy1_inv_mfma", io});
+    return split;
+  }
   const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
   const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
   if (plan == CF_PLAN_PLANE3) {
@@ -581,7 +602,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
     release_all();
     return -4;
   }
-  if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
+  if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.dft16gen, "cf_dft16_table_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
       get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
       get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_reset, "cf_batch_reset") != CF_OK ||
       ensure_events() != CF_OK) {
@@ -788,7 +809,14 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
   if (h->count != 1 && h->count != how_many)
     return fail(CF_ERR_ARG, "spectrum holds %u filters; need 1 or how_many (%u)", h->count, how_many);
   if (scale_mode != CF_SCALE_NONE && scale_mode != CF_SCALE_INV_N3) return fail(CF_ERR_ARG, "bad scale_mode");
-  if (precision != CF_PREC_FP32) return fail(CF_ERR_ARG, "precision %d is not available", precision);
+  if (precision != CF_PREC_FP32 && precision != CF_PREC_SPLIT_FP16)
+    return fail(CF_ERR_ARG, "unknown precision %d", precision);
+  if (precision == CF_PREC_SPLIT_FP16 &&
+      (h->n != 256 || !is_split(h->plan) || cfgeo::split_gm(split_variant(h->plan)) != 2))
+    return fail(CF_ERR_ARG,
+                "split-FP16 precision runs N = 256 with a spectrum of a four-step plan with middle tiles of 2 x "
+                "(split3_g22, g42, g24m2, g28m2); this spectrum is N = %u, plan %d",
+                h->n, h->plan);
   CFCHK(require_ready());
   SizeFns* f = nullptr;
   CFCHK(size_res(h->n, &f));
@@ -799,7 +827,7 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
   void* work = nullptr;
   if (is_split(h->plan)) CFCHK(acquire_work(cube_bytes(n) * how_many, s, &work));
   std::vector<Pass> p;
-  conv_passes(*f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p);
+  conv_passes(*f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p, precision);
   struct WorkRelease {  // the workspace is marked busy on s whatever path returns
     void* w;
     hipStream_t s;

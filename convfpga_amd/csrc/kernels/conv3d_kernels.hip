@@ -14,6 +14,7 @@
 #include "batch_kernel.h"
 #include "fft_core.h"
 #include "three_pass.h"
+#include "split_mfma.h"
 #include "../geometry.h"
 
 namespace cfk {
@@ -117,6 +118,27 @@ static_assert(SplitGeoCheck<0>::ok && SplitGeoCheck<1>::ok && SplitGeoCheck<2>::
                   SplitGeoCheck<4>::ok && SplitGeoCheck<5>::ok && SplitGeoCheck<6>::ok && SplitGeoCheck<7>::ok &&
                   SplitGeoCheck<8>::ok && cfgeo::kSplitVariants == 9,
               "");
+
+// ---- split-FP16 MFMA form of the four-step passes (split_mfma.h) ----------------------
+// outer passes GA = 2 over an intermediate of x group 4, middle tiles of 2 x:
+// the spectrum layout of the GM = 2 plans
+extern "C" __global__ __launch_bounds__(512) void cf_xy1_fwd_256_s24(CF_ARGS) {
+  cfk::xy1_mfma_kernel<2, 4, false>(in, out, tw, scale, ntotal);
+}
+extern "C" __global__ __launch_bounds__(512) void cf_xy1_inv_256_s24(CF_ARGS) {
+  cfk::xy1_mfma_kernel<2, 4, true>(in, out, tw, scale, ntotal);
+}
+extern "C" __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void cf_mid_256_s24m2(CF_ARGS) {
+  cfk::mid_mfma_kernel<4, 2>(in, out, H, tw, hstride, ntotal);
+}
+static_assert(cfk::Xy1MfmaGeo<2, 4>::THREADS == 512 && cfk::MidMfmaGeo<4, 2>::THREADS == 512, "split-FP16 geometry");
+
+// The A operands of the 16-point DFT products (hi and lo FP16 parts of W/16
+// and conj(W)/16, per lane) from double-precision sincospi: 128 threads.
+extern "C" __global__ void cf_dft16_table_gen(void* table) {
+  const int t = threadIdx.x;  // (inv, lane)
+  if (t < 128) cfk::dft16_table_entry(t / 64, t % 64, reinterpret_cast<cfk::h8_t*>(table) + t * 4);
+}
 
 // ---- persistent batch of mixed sizes (batch_kernel.h) -----------------------------
 extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(

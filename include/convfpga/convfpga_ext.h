@@ -31,7 +31,12 @@ extern "C" {
 
 /* arithmetic of the radix-16 DFT stages */
 #define CF_PREC_FP32 0        /* FP32 FMA butterflies                        */
-#define CF_PREC_SPLIT_FP16 1  /* MFMA complex GEMM, split hi+lo FP16 operands */
+#define CF_PREC_SPLIT_FP16 1  /* 16-point DFTs as complex GEMMs on the matrix cores:
+                               * hi+lo FP16 operands, three products, FP32
+                               * accumulation, one power-of-two scale per tile.
+                               * N = 256 with a spectrum of a four-step plan
+                               * whose middle tiles are 2 x wide (split3_g22,
+                               * g42, g24m2, g28m2); other spectra: CF_ERR_ARG */
 
 /* execution plans (which passes and tiles a convolution of edge N runs) */
 #define CF_PLAN_AUTO 0       /* the autotuner's choice (default)                              */

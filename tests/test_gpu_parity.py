@@ -396,3 +396,66 @@ def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle):
     d = _t(oracle.uniform(256 ** 3, 72002))
     with pytest.raises(gpu.ConvError, match="four-step"):
         api.Batch([(d, d, spec)], mode=api.CF_BATCH_PERSISTENT)
+
+
+REL_L2_SPLIT = 1e-5  # stated bound of the split-FP16 MFMA form against the FP64 oracle
+
+
+@pytest.mark.parametrize("dist", ["uniform", "loguniform"])
+def test_split_fp16_256_matches_oracle(gpu, oracle, dist):
+    """Split-FP16 MFMA form at 256^3, two instances with their own spectra:
+    rel-L2 against the FP64 oracle within the stated bound, for U[-1,1) and
+    for log-uniform magnitudes (1e-3..1e3, uniform phase); in place gives
+    the same bits."""
+    import torch
+    n, b = 256, 2
+    P = n ** 3
+    print("case split-FP16 N=256 %s" % dist, flush=True)
+    api = gpu.api
+    gen = oracle.uniform if dist == "uniform" else oracle.loguniform
+    sig, filt = gen(b * P, 18001), gen(b * P, 18002)
+    api.set_plan(n, api.CF_PLAN_SPLIT3_G24M2)
+    try:
+        spec = gpu.Spectrum(n, _t(filt), count=b)
+        ds = _t(sig)
+        d16, d32 = torch.empty_like(ds), torch.empty_like(ds)
+        gpu.conv3d_device(spec, ds, d16, how_many=b, precision=api.CF_PREC_SPLIT_FP16)
+        gpu.conv3d_device(spec, ds, d32, how_many=b)
+        g16, g32 = _np(d16), _np(d32)
+        for i in range(b):
+            s = slice(i * P, (i + 1) * P)
+            ref = oracle.conv3d_f64(n, np.ascontiguousarray(sig[s]), np.ascontiguousarray(filt[s]), 1.0 / P)
+            e16 = oracle.rel_l2(np.ascontiguousarray(g16[s]), ref)
+            e32 = oracle.rel_l2(np.ascontiguousarray(g32[s]), ref)
+            print("  instance %d: rel-L2 split-FP16 %.3e, FP32 %.3e" % (i, e16, e32), flush=True)
+            assert e16 <= REL_L2_SPLIT
+            assert e32 <= REL_L2_FP32
+        gpu.conv3d_device(spec, ds, ds, how_many=b, precision=api.CF_PREC_SPLIT_FP16)
+        assert np.array_equal(_np(ds), g16)
+        del spec, ds, d16, d32
+        torch.cuda.empty_cache()
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+
+
+def test_split_fp16_rejects_other_layouts(gpu, oracle):
+    import torch
+    print("case split-FP16 argument checks", flush=True)
+    api = gpu.api
+    f64 = _t(oracle.uniform(64 ** 3, 5))
+    s64 = gpu.Spectrum(64, f64)
+    x = torch.empty_like(f64)
+    with pytest.raises(api.ConvError):
+        gpu.conv3d_device(s64, f64, x, precision=api.CF_PREC_SPLIT_FP16)
+    api.set_plan(256, api.CF_PLAN_SPLIT3_G24)  # middle tiles of 4 x: another spectrum layout
+    try:
+        f = _t(oracle.uniform(256 ** 3, 6))
+        s = gpu.Spectrum(256, f)
+        y = torch.empty_like(f)
+        with pytest.raises(api.ConvError):
+            gpu.conv3d_device(s, f, y, precision=api.CF_PREC_SPLIT_FP16)
+        with pytest.raises(api.ConvError):
+            gpu.conv3d_device(s, f, y, precision=7)
+        del s, f, y
+    finally:
+        api.set_plan(256, api.CF_PLAN_AUTO)
