@@ -19,7 +19,10 @@ against the FP64 oracle.
 Other workloads (reported lines, not the metric): c2_128 / c1_64 (one cube),
 c3_64x64 (64 cubes of 64^3, one shared spectrum), c4_mixed (16x32^3 + 8x64^3 +
 4x128^3 + 1x256^3, own spectra, shuffled), each comparing the persistent
-one-launch kernel, the recorded graph and, where it applies, batched passes.
+one-launch kernel, the recorded graph and, where it applies, batched passes;
+c5_split (4x256^3 + 1x512^3, own spectra, U[-1,1) and log-uniform inputs): the
+split-FP16 MFMA form beside the FP32 form, rel-L2 of every instance against
+the FP64 oracle.
 """
 import argparse
 import json
@@ -36,6 +39,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 
 SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64}
+SPLIT = {"c5_split": [256] * 4 + [512]}
 BATCHES = {
     "c3_64x64": [64] * 64,
     "c4_mixed": [32] * 16 + [64] * 8 + [128] * 4 + [256],
@@ -47,7 +51,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2_256", choices=sorted(SINGLE) + sorted(BATCHES))
+    ap.add_argument("--workload", default="c2_256", choices=sorted(SINGLE) + sorted(BATCHES) + sorted(SPLIT))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--plan", type=int, default=0, help="force a schedule (CF_PLAN_*); 0 = autotuned")
@@ -60,7 +64,8 @@ def parse():
 def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
     """Times the oracle's FP32 CPU convolution (port of src/fft_conv3D.cpp:50-177)
     on `reps` convolutions of one cube, and checks the GPU output on the same
-    inputs.  The only place bench.py uses the oracle."""
+    inputs.  With split_check_leg (the c5 checker), the only places bench.py
+    uses the oracle."""
     import numpy as np
     from oracle import oracle as O
     O.build()
@@ -312,6 +317,113 @@ def run_batch(a, torch, C, api):
     return line, None
 
 
+def run_split(a, torch, C, api):
+    """4 x 256^3 + 1 x 512^3 convolutions with their own filters, for U[-1,1)
+    and for log-uniform inputs (both signal and filter): the split-FP16 MFMA
+    form beside the FP32 form.  The MFMA form exists for the 256^3 four-step
+    kernels; the 512^3 instance runs the FP32 5-pass schedule in both columns.
+    A step is the whole batch (one call for the four 256^3 cubes, one for the
+    512^3 cube), inputs resident in HBM, filter spectra precomputed."""
+    n1, b1, n2 = 256, 4, 512
+    P1, P2 = n1 ** 3, n2 ** 3
+    api.set_plan(n1, api.CF_PLAN_SPLIT3_G24M2)  # spectrum layout read by both forms
+    dists = {}
+    checks = {}
+    for di, dist in enumerate(("uniform", "loguniform")):
+        seed = a.seed + 100 * di
+        sig1 = torch.empty(b1 * P1, dtype=torch.complex64, device="cuda")
+        flt1 = torch.empty_like(sig1)
+        sig2 = torch.empty(P2, dtype=torch.complex64, device="cuda")
+        flt2 = torch.empty_like(sig2)
+        for k, x in enumerate((sig1, flt1, sig2, flt2)):
+            if dist == "uniform":
+                api.fill_uniform(x, 2 * x.numel(), seed + k)
+            else:
+                api.fill_loguniform(x, x.numel(), seed + k)
+        torch.cuda.synchronize()
+        spec1 = C.Spectrum(n1, flt1, count=b1)
+        spec2 = C.Spectrum(n2, flt2)
+        out1, out2 = torch.empty_like(sig1), torch.empty_like(sig2)
+        entry = {}
+        host = {"sig256": sig1.cpu().numpy(), "flt256": flt1.cpu().numpy(),
+                "sig512": sig2.cpu().numpy(), "flt512": flt2.cpu().numpy()}
+        for prec, pname in ((api.CF_PREC_FP32, "fp32"), (api.CF_PREC_SPLIT_FP16, "split_fp16_mfma")):
+            def part1():
+                C.conv3d_device(spec1, sig1, out1, how_many=b1, precision=prec)
+
+            def part2():
+                C.conv3d_device(spec2, sig2, out2)
+
+            def timed(fn, steps):
+                for _ in range(a.warmup):
+                    fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    fn()
+                torch.cuda.synchronize()
+                return (time.perf_counter() - t0) / steps
+
+            t_batch = timed(lambda: (part1(), part2()), a.steps)
+            t1 = timed(part1, a.steps)
+            t2 = timed(part2, max(1, a.steps // 4))
+            entry[pname] = {"ms_per_batch": 1e3 * t_batch, "conv_per_s": (b1 + 1) / t_batch,
+                            "ms_4x256": 1e3 * t1, "ms_512": 1e3 * t2,
+                            "conv_per_s_256": b1 / t1, "ms_per_256_conv": 1e3 * t1 / b1}
+            part1()
+            part2()
+            host["out256_" + pname] = out1.cpu().numpy()
+            host["out512_" + pname] = out2.cpu().numpy()
+        entry["split_vs_fp32_speed_256"] = entry["fp32"]["ms_4x256"] / entry["split_fp16_mfma"]["ms_4x256"]
+        dists[dist] = entry
+        checks[dist] = host
+        del spec1, spec2, sig1, flt1, sig2, flt2, out1, out2
+        torch.cuda.empty_cache()
+    api.set_plan(n1, api.CF_PLAN_AUTO)
+    V1, V2 = 8.0 * P1, 8.0 * P2
+    alg = b1 * 7 * V1 + 11 * V2  # executed schedules: four-step 256^3 (7V), 5-pass 512^3 (11V)
+    u = dists["uniform"]["split_fp16_mfma"]
+    line = base_line(a, "split-MFMA vs FP32 FFT convolution, 4x256^3 + 1x512^3 (own filters)", u["conv_per_s"],
+                     "conv/s", u["ms_per_batch"],
+                     {"workload": "c5_split_4x256^3+1x512^3", "instances": b1 + 1, "filters": "own per instance",
+                      "scale": "1/N^3", "plans": {"256": "split3_g24m2", "512": "axis5"},
+                      "split_fp16_sizes": [256]})
+    line["dtype"] = "split-fp16 mfma (fp32 accumulate) for 256^3; fp32 for 512^3"
+    line["data"] = "synthetic: U[-1,1) and log-uniform magnitudes 1e-3..1e3 with uniform phase, generated on the GPU"
+    line["by_distribution"] = {d: {k: ({kk: round(vv, 5) for kk, vv in v.items()} if isinstance(v, dict) else round(v, 4))
+                                   for k, v in e.items()} for d, e in dists.items()}
+    line["roofline"] = {"bound": "hbm", "kernel": "whole batch (split-FP16 256^3 + FP32 512^3)",
+                        "achieved": alg / (u["ms_per_batch"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (u["ms_per_batch"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_model": "4 x 7V (256^3 four-step) + 11V (512^3 five-pass)"}
+    return line, checks
+
+
+def split_check_leg(checks, reps):
+    """Checker and CPU baseline of the c5 workload: rel-L2 of every instance
+    of both forms against the FP64 oracle on the very inputs the GPU used
+    (copied back), and the oracle's FP32 CPU convolution timed at 256^3."""
+    import numpy as np
+    from oracle import oracle as O
+    O.build()
+    acc = {}
+    for dist, h in checks.items():
+        e = {"fp32": [], "split_fp16_mfma": []}
+        for inst, (n, sk, fk, b) in enumerate([(256, "sig256", "flt256", k) for k in range(4)] + [(512, "sig512", "flt512", 0)]):
+            P = n ** 3
+            sig = np.ascontiguousarray(h[sk][b * P:(b + 1) * P])
+            flt = np.ascontiguousarray(h[fk][b * P:(b + 1) * P])
+            ref = O.conv3d_f64(n, sig, flt, 1.0 / P)
+            for form in e:
+                got = np.ascontiguousarray(h["out%d_%s" % (n, form)][b * P:(b + 1) * P])
+                e[form].append(O.rel_l2(got, ref))
+            del ref
+        acc[dist] = {k: [float("%.3e" % x) for x in v] for k, v in e.items()}
+        acc[dist]["instances"] = ["256^3"] * 4 + ["512^3 (fp32 in both columns)"]
+    base, _ = cpu_baseline_leg(256, None, 1, 2, reps)
+    return base, {"rel_l2_vs_fp64": acc}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -327,16 +439,20 @@ def main():
     rc = C.fpga_initialize("gfx950", C.CODE_OBJECT, True)
     if rc != 0:
         raise SystemExit("fpga_initialize -> %d" % rc)
-    if a.workload in SINGLE:
-        line, cpu = run_single(a, torch, C, api)
-    else:
-        line, cpu = run_batch(a, torch, C, api)
-    if cpu is not None and not a.no_cpu_baseline:
-        base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps)
-        line["cpu_baseline"] = base
-        line.update(check)
-    else:
+    if a.workload in SPLIT:
+        line, checks = run_split(a, torch, C, api)
         line["cpu_baseline"] = None
+        if not a.no_cpu_baseline:
+            base, check = split_check_leg(checks, a.cpu_reps)
+            line["cpu_baseline"] = base
+            line.update(check)
+    else:
+        line, cpu = (run_single if a.workload in SINGLE else run_batch)(a, torch, C, api)
+        line["cpu_baseline"] = None
+        if cpu is not None and not a.no_cpu_baseline:
+            base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps)
+            line["cpu_baseline"] = base
+            line.update(check)
     C.fpga_final()
     print(json.dumps(line), flush=True)
     return 0

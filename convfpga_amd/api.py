@@ -40,7 +40,7 @@ DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpga
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
                "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_conv3d_device",
                "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
-               "cf_profile_end", "cf_fill_uniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
+               "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
                "cf_batch_destroy")
 
@@ -105,6 +105,7 @@ def lib():
             "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
             "cf_profile_begin": (i, [u]),
             "cf_fill_uniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
+            "cf_fill_loguniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
             "cf_plan_supported": (i, [u, i]),
             "cf_set_plan": (i, [u, i]),
             "cf_get_plan": (i, [u]),
@@ -278,6 +279,10 @@ def profile_end():
 def fill_uniform(x, nfloats, seed, stream=None):
     """Device-side U[-1,1) fill, bit-identical to the host splitmix64 generator."""
     _check(lib().cf_fill_uniform(_dev(x), nfloats, seed, _stream(stream)), "cf_fill_uniform")
+
+
+def fill_loguniform(x, npts, seed, stream=None):
+    _check(lib().cf_fill_loguniform(_dev(x), npts, seed, _stream(stream)), "cf_fill_loguniform")
 
 
 def plan_supported(N, plan):

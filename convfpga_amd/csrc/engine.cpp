@@ -88,7 +88,7 @@ struct Engine {
   int device = 0;
   hipModule_t mod = nullptr;
   hipStream_t stream = nullptr;
-  hipFunction_t twgen = nullptr, dft16gen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr;
+  hipFunction_t twgen = nullptr, dft16gen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr, filll = nullptr;
   hipFunction_t batch_run = nullptr, batch_reset = nullptr;
   hipStream_t capture = nullptr;  // private stream for graph capture
   std::map<unsigned, SizeFns> sizes;
@@ -167,7 +167,7 @@ void release_all() {
   g.stream = nullptr;
   if (g.mod) hipModuleUnload(g.mod);
   g.mod = nullptr;
-  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.batch_run = g.batch_reset = nullptr;
+  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_reset = nullptr;
   if (g.capture) hipStreamDestroy(g.capture);
   g.capture = nullptr;
   g.ready = false;
@@ -604,6 +604,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
   }
   if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.dft16gen, "cf_dft16_table_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
       get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
+      get_fn(&g.filll, "cf_fill_loguniform") != CF_OK ||
       get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_reset, "cf_batch_reset") != CF_OK ||
       ensure_events() != CF_OK) {
     fprintf(stderr, "convfpga: '%s' is not a convfpga code object (%s)\n", path, g_err.c_str());
@@ -969,6 +970,18 @@ extern "C" int cf_fill_uniform(void* d_out, size_t nfloats, unsigned long long s
   HIPCHK(hipModuleLaunchKernel(g.fillu, blocks, 1, 1, 256, 1, 1, 0, (hipStream_t)stream, args, nullptr));
   return CF_OK;
 }
+extern "C" int cf_fill_loguniform(void* d_out, size_t npts, unsigned long long seed, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!d_out || npts == 0) return fail(CF_ERR_ARG, "NULL pointer or zero size");
+  CFCHK(require_ready());
+  long long np = (long long)npts;
+  unsigned long long sd = seed;
+  const unsigned blocks = (unsigned)std::min<size_t>(4096, (npts + 255) / 256);
+  void* args[] = {&d_out, &np, &sd};
+  HIPCHK(hipModuleLaunchKernel(g.filll, blocks, 1, 1, 256, 1, 1, 0, (hipStream_t)stream, args, nullptr));
+  return CF_OK;
+}
+
 // ---- batches -------------------------------------------------------------------
 struct cf_batch_s {
   int mode = CF_BATCH_PERSISTENT;

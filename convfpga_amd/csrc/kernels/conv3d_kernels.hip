@@ -245,3 +245,26 @@ extern "C" __global__ __launch_bounds__(256) void cf_fill_uniform(float* out, lo
     out[i] = (float)(k - (1 << 23)) * (1.0f / 8388608.0f);
   }
 }
+
+// Point p = (m cos(ph), m sin(ph)) with m = 10^(6 u1 - 3), ph = 2 pi u2, u1 and
+// u2 the top 24 bits of splitmix64(seed + (2p+1)*golden) and of (2p+2)/2^24,
+// evaluated in double: log-uniform magnitudes in [1e-3, 1e3), uniform phase.
+__device__ __forceinline__ unsigned long long cf_mix(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+extern "C" __global__ __launch_bounds__(256) void cf_fill_loguniform(float2* out, long long npts,
+                                                                     unsigned long long seed) {
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < npts; p += stride) {
+    const double u1 = (double)(cf_mix(seed, 2 * p) >> 40) / 16777216.0;
+    const double u2 = (double)(cf_mix(seed, 2 * p + 1) >> 40) / 16777216.0;
+    const double mag = pow(10.0, 6.0 * u1 - 3.0);
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    out[p] = make_float2((float)(mag * c), (float)(mag * s));
+  }
+}
+

@@ -155,6 +155,10 @@ int cf_rel_l2_device(const void *d_test, const void *d_ref, int ref_is_f64, size
  * (value i = k/2^23 - 1, k = top 24 bits of splitmix64(seed + (i+1)*0x9E3779B97F4A7C15)),
  * written on the device; asynchronous on `stream`. */
 int cf_fill_uniform(void *d_out, size_t nfloats, unsigned long long seed, void *stream);
+/* npts complex points with log-uniform magnitudes 10^(6 u1 - 3) in [1e-3, 1e3)
+ * and uniform phase 2 pi u2 (u1, u2: the 24-bit splitmix64 variates 2p and
+ * 2p+1 of the same generator), computed in double on the device. */
+int cf_fill_loguniform(void *d_out, size_t npts, unsigned long long seed, void *stream);
 
 #ifdef __cplusplus
 }

@@ -208,6 +208,15 @@ def test_device_fill_matches_host_generator(gpu, oracle):
     gpu.api.fill_uniform(x, 2 * x.numel(), 987654321)
     want = oracle.uniform(x.numel(), 987654321)
     assert np.array_equal(_np(x).view(np.uint32), want.view(np.uint32))
+    # log-uniform: same variates; pow/sin/cos of the device and host math
+    # libraries may differ in the last place of the double, so the floats
+    # agree to a few ulp
+    gpu.api.fill_loguniform(x, x.numel(), 424242)
+    want = oracle.loguniform(x.numel(), 424242)
+    got = _np(x)
+    assert np.all(np.abs(got - want) <= 4e-7 * np.abs(want))
+    m = np.abs(got)
+    assert 0.9e-3 < m.min() and m.max() < 1.01e3
 
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
