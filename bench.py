@@ -237,22 +237,30 @@ def run_batch(a, torch, C, api):
         api.fill_uniform(sig[n], 2 * cnt * P, a.seed + n)
         api.fill_uniform(filt[n], filt[n].numel() * 2, a.seed + 7 * n)
     torch.cuda.synchronize()
-    # spectra in natural order (the persistent kernel's layout): axis plan for 256
+    def make_specs():
+        return {n: [C.Spectrum(n, filt[n][k * n ** 3:(k + 1) * n ** 3]) for k in range(1 if shared else cnt)]
+                for n, cnt in by_n.items()}
+
+    def instances(specs):
+        used = {n: 0 for n in by_n}
+        res = []
+        for n in sizes:  # in the (shuffled) instance order
+            k = used[n]
+            used[n] += 1
+            P = n ** 3
+            res.append((sig[n][k * P:(k + 1) * P], out[n][k * P:(k + 1) * P], specs[n][0 if shared else k]))
+        return res
+
+    # graph mode: spectra of each edge's autotuned plan; persistent mode:
+    # spectra in natural order (its layout), i.e. the axis plan for 256
+    inst_g = instances(make_specs())
     if 256 in by_n:
         api.set_plan(256, api.CF_PLAN_AXIS5)
-    specs = {}
-    for n, cnt in by_n.items():
-        P = n ** 3
-        specs[n] = [C.Spectrum(n, filt[n][k * P:(k + 1) * P]) for k in range(1 if shared else cnt)]
-    used = {n: 0 for n in by_n}
-    inst = []
-    for n in sizes:  # in the (shuffled) instance order
-        k = used[n]
-        used[n] += 1
-        P = n ** 3
-        inst.append((sig[n][k * P:(k + 1) * P], out[n][k * P:(k + 1) * P], specs[n][0 if shared else k]))
+    inst = instances(make_specs())
+    if 256 in by_n:
+        api.set_plan(256, api.CF_PLAN_AUTO)
     # graph mode records contiguous runs of one spectrum as batched passes: order by edge
-    inst_sorted = sorted(inst, key=lambda t: (t[2].N, t[0].data_ptr()))
+    inst_sorted = sorted(inst_g, key=lambda t: (t[2].N, t[0].data_ptr()))
     results = {}
 
     def timed(launch, sync_check):
@@ -277,12 +285,15 @@ def run_batch(a, torch, C, api):
         results["batched_passes_" + api.PLAN_NAMES[sp.plan]] = timed(
             lambda: C.conv3d_device(sp, sig[64], out[64], how_many=len(sizes)), torch.cuda.synchronize)
     per_size = {}
-    if not shared:  # per-edge throughput: each edge group alone in one persistent launch
-        for n, cnt in sorted(by_n.items()):
-            b = api.Batch([(s, o, h) for s, o, h in inst if h.N == n], mode=api.CF_BATCH_PERSISTENT)
-            w = timed(b.launch, b.check)
-            per_size["%d^3 x%d" % (n, cnt)] = {"conv_per_s": cnt / w, "ms": 1e3 * w}
-            b.close()
+    if not shared:  # per-edge throughput: each edge group alone, in each mode
+        for mode, mname, group in ((api.CF_BATCH_GRAPH, "graph", inst_g),
+                                   (api.CF_BATCH_PERSISTENT, "persistent_one_launch", inst)):
+            per_size[mname] = {}
+            for n, cnt in sorted(by_n.items()):
+                b = api.Batch([(s, o, h) for s, o, h in group if h.N == n], mode=mode)
+                w = timed(b.launch, b.check)
+                per_size[mname]["%d^3 x%d" % (n, cnt)] = {"conv_per_s": cnt / w, "ms": 1e3 * w}
+                b.close()
     best = min(results, key=lambda k: results[k])
     wall = results[best]
 
@@ -296,6 +307,7 @@ def run_batch(a, torch, C, api):
     elif best.startswith("batched"):
         passes_of = {id(h): npass(sp.plan) for _, _, h in inst}
     else:
+        inst = inst_g
         passes_of = {id(h): npass(h.plan) for _, _, h in inst}
     alg = sum(2 * passes_of[id(h)] * 8.0 * h.N ** 3 for _, _, h in inst)
     alg += sum(8.0 * h.N ** 3 for h in {id(h): h for _, _, h in inst}.values())
@@ -312,8 +324,6 @@ def run_batch(a, torch, C, api):
         line["per_size"] = per_size
     bp.close()
     bg.close()
-    if 256 in by_n:
-        api.set_plan(256, api.CF_PLAN_AUTO)
     return line, None
 
 
