@@ -260,13 +260,14 @@ struct MidGeo {
 };
 
 // Middle pass on tile (k1, x0 .. x0+GM-1).
-template <int GA, int GB, int GM>
+template <int GA, int GB, int GM, bool HPF = false>
 struct MidPass {
   using K = Split256<GA, GB>;
   using M = MidGeo<GB, GM>;
   static constexpr int N = 256, L = M::LINES;
   struct Regs {
     float2 v[16];
+    float2 hv[HPF ? 16 : 1];  // HPF: the spectrum values, loaded with the points
   };
   const float2* in;
   float2* out;
@@ -284,10 +285,18 @@ struct MidPass {
     const size_t cube = (size_t)(tile / M::TILES) * N * N * N;
     return cube + K::mid_index(x0 + l % GM, tt / M::NXM, l / GM, zt);
   }
+  __device__ __forceinline__ const float2* h_ptr(int tile) const {
+    return H + (size_t)(tile / M::TILES) * hstride + (size_t)(tile % M::TILES) * M::HTILE + qz * GM + qx;
+  }
   __device__ __forceinline__ void load(int tile, Regs& r) const {
     const float2* src = in + thread_off(tile);
 #pragma unroll
     for (int m = 0; m < 16; ++m) r.v[m] = src[256 * GB * m];
+    if constexpr (HPF) {
+      const float2* h = h_ptr(tile);
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) r.hv[k2] = h[k2 * 256 * GM];
+    }
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
     constexpr int PITCH = M::PITCH;
@@ -300,11 +309,15 @@ struct MidPass {
     float2* rowp = lds + qz * PITCH + qx;
 #pragma unroll
     for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GM];
-    const int cube = tile / M::TILES;
-    const float2* h = H + (size_t)cube * hstride + (size_t)(tile % M::TILES) * M::HTILE + qz * GM + qx;
     float2 hv[16];
+    if constexpr (HPF) {
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GM];
+      for (int k2 = 0; k2 < 16; ++k2) hv[k2] = r.hv[k2];
+    } else {
+      const float2* h = h_ptr(tile);
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GM];
+    }
     DFT<16, false>::run(u);  // u[k2]: full 3D spectrum at (x, k1 + 16 k2, qz)
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) u[k2] = cmul(u[k2], hv[k2]);
@@ -323,7 +336,7 @@ struct MidPass {
   }
 };
 
-template <int GA, int GB, int GM>
+template <int GA, int GB, int GM, bool HPF = false>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
   using M = MidGeo<GB, GM>;
@@ -332,11 +345,11 @@ __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const 
   __shared__ float2 tw_s[256];
   stage_twiddles<256, M::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
-  const MidPass<GA, GB, GM> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x % GM, x / GM};
+  const MidPass<GA, GB, GM, HPF> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x % GM, x / GM};
   // siblings (consecutive tiles of one x group) on one XCD, back to back
   const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
   if (tile >= ntotal) return;
-  typename MidPass<GA, GB, GM>::Regs a;
+  typename MidPass<GA, GB, GM, HPF>::Regs a;
   k.load(tile, a);
   k.process(tile, a);
 }
