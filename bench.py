@@ -190,10 +190,35 @@ def run_single(a, torch, C, api):
     line["copy_kernel_GBps"] = copy_gbs
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
+    line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, out)
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
     return line, (n, host_out, seed_sig, seed_filt)
+
+
+def own_filter_leg(torch, C, n, sig, filt, out, reps=20):
+    """The convolution with its own filter (device-resident): filter transform
+    (spectrum creation, including its allocation) plus the convolution, per
+    call, and the filter setup alone.  Reported beside the metric, whose
+    spectrum is precomputed."""
+    def one():
+        sp = C.Spectrum(n, filt)
+        C.conv3d_device(sp, sig, out)
+        sp.close()
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        C.Spectrum(n, filt).close()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"ms_per_conv_with_filter_transform": 1e3 * (t1 - t0) / reps,
+            "filter_setup_ms": 1e3 * (t2 - t1) / reps, "reps": reps}
 
 
 def dropin_host_leg(C, n, sig, filt, reps=3):

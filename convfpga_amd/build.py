@@ -1,12 +1,14 @@
 """Build libconvfpga for gfx950, in-tree.
 
-Two artefacts under convfpga_amd/lib/:
+Three artefacts under convfpga_amd/lib/:
   convfpga_gfx950.hsaco  the kernel code object (hipcc --genco), loaded by
                          fpga_initialize(path) -- the analogue of the
                          reference's .aocx bitstream;
   libconvfpga.so         the C-ABI host library (g++, HIP runtime API only),
                          exporting include/convfpga/convfpga.h and
-                         include/convfpga/convfpga_ext.h.
+                         include/convfpga/convfpga_ext.h;
+  convolution            the command-line harness (cli/convolution.cpp), a
+                         plain C++ client of libconvfpga.so.
 Run as `python -m convfpga_amd.build` or through __graft_entry__.build().
 """
 import os
@@ -24,6 +26,8 @@ ARCH = "gfx950"
 
 HSACO = os.path.join(LIBDIR, "convfpga_%s.hsaco" % ARCH)
 SHLIB = os.path.join(LIBDIR, "libconvfpga.so")
+CLI = os.path.join(LIBDIR, "convolution")
+CLI_SOURCES = [os.path.join(PKG, "cli", "convolution.cpp")]
 
 KERNEL_SOURCES = [os.path.join(CSRC, "kernels", "conv3d_kernels.hip")]
 HOST_SOURCES = [os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "capi.cpp")]
@@ -56,6 +60,10 @@ def build(verbose=False, extra_hip_flags=(), hsaco=None):
          + HOST_SOURCES + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
                            "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", tmp])
     os.replace(tmp, SHLIB)
+    tmp = CLI + ".tmp"
+    _run([cxx, "-O2", "-std=c++17", "-fopenmp", "-Wall", "-Wextra", "-I" + INCLUDE] + CLI_SOURCES
+         + ["-L" + LIBDIR, "-lconvfpga", "-ldl", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, CLI)
     if verbose:
         print("built", HSACO, "and", SHLIB)
     return SHLIB, HSACO
