@@ -55,7 +55,8 @@ def build(verbose=False, extra_hip_flags=(), hsaco=None):
     os.replace(tmp, target)
     cxx = os.environ.get("CXX", "g++")
     tmp = SHLIB + ".tmp"
-    _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter",
+    _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-fvisibility=hidden", "-Wall", "-Wextra",
+          "-Wno-unused-parameter",
           "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), "-I" + INCLUDE, "-I" + CSRC]
          + HOST_SOURCES + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
                            "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", tmp])
