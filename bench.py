@@ -263,8 +263,17 @@ def run_batch(a, torch, C, api):
         api.fill_uniform(filt[n], filt[n].numel() * 2, a.seed + 7 * n)
     torch.cuda.synchronize()
     def make_specs():
-        return {n: [C.Spectrum(n, filt[n][k * n ** 3:(k + 1) * n ** 3]) for k in range(1 if shared else cnt)]
-                for n, cnt in by_n.items()}
+        # one multi-filter spectrum per edge, one view per instance: the graph
+        # batch records consecutive views with consecutive signals as one
+        # batched pass sequence
+        specs = {}
+        for n, cnt in by_n.items():
+            if shared:
+                specs[n] = [C.Spectrum(n, filt[n])]
+            else:
+                parent = C.Spectrum(n, filt[n], count=cnt)
+                specs[n] = [parent.view(k) for k in range(cnt)]
+        return specs
 
     def instances(specs):
         used = {n: 0 for n in by_n}

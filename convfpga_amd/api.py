@@ -38,7 +38,7 @@ SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpgaf_conv3D",
                   "fpgaf_conv3D_svm", "fpgaf_conv3D_svm_batch")
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
-               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_conv3d_device",
+               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device",
                "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
@@ -99,6 +99,7 @@ def lib():
             "cf_spectrum_data": (vp, [vp]),
             "cf_spectrum_edge": (u, [vp]),
             "cf_spectrum_count": (u, [vp]),
+            "cf_spectrum_view": (i, [vp, u, ctypes.POINTER(vp)]),
             "cf_conv3d_device": (i, [vp, vp, vp, u, i, i, vp]),
             "cf_conv3d_device_timed": (i, [vp, vp, vp, u, i, i, vp, ctypes.POINTER(cf_pass_times_t)]),
             "cf_fft3d_device": (i, [u, vp, vp, u, i, vp]),
@@ -207,6 +208,7 @@ class Spectrum:
     """Precomputed filter spectra (cf_spectrum_t): count filters of edge N."""
 
     def __init__(self, N, filters, count=1, on_device=True, stream=None):
+        self._parent = None
         h = ctypes.c_void_p()
         if on_device:
             ptr = _dev(filters)
@@ -216,6 +218,19 @@ class Spectrum:
                "cf_spectrum_create")
         self._h = h
         self.N, self.count = N, count
+
+    @classmethod
+    def _wrap(cls, h, N, parent):
+        obj = cls.__new__(cls)
+        obj._h, obj.N, obj.count, obj._parent = h, N, 1, parent
+        return obj
+
+    def view(self, index):
+        """Spectrum handle for filter `index` alone, sharing this one's memory
+        (cf_spectrum_view); it keeps this object alive."""
+        h = ctypes.c_void_p()
+        _check(lib().cf_spectrum_view(self._h, index, ctypes.byref(h)), "cf_spectrum_view")
+        return Spectrum._wrap(h, self.N, self)
 
     @property
     def handle(self):

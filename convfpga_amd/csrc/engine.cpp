@@ -779,8 +779,8 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   CFCHK(ensure_pinned(&g.h_out, &g.h_out_bytes, SV));
   // host copies into the mapped buffers (the reference's SVM map + memcpy,
   // convfpga_svm.c:86-103); the first and last passes then use them over the bus.
-  memcpy(g.h_filt, filter, V);
-  memcpy(g.h_sig, sig, SV);
+  host_copy(g.h_filt, filter, V);
+  host_copy(g.h_sig, sig, SV);
   spectrum_passes(*f, plan, n, 1, mapped(g.h_filt), g.d_filt, work, filt_p);
   const size_t nf = filt_p.size();
   CFCHK(run_passes(filt_p, n, 1, s, f->tw, ev));  // ev[0 .. nf]
@@ -791,7 +791,7 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   CFCHK(run_passes(conv_p, n, how_many, s, f->tw, ec));  // ec[0 .. nc]
   if (work) CFCHK(release_work(s));
   HIPCHK(hipStreamSynchronize(s));
-  memcpy(out, g.h_out, SV);
+  host_copy(out, g.h_out, SV);
   t->filter_exec_t = ms_between(ev[0], ev[nf]);
   if (how_many == 1) {
     t->sig_exec_t = ms_between(ec[0], ec[split]);
@@ -813,6 +813,7 @@ struct cf_spectrum_s {
   unsigned n = 0, count = 0;
   int plan = CF_PLAN_AXIS5;
   void* d = nullptr;
+  bool view = false;  // aliases one filter of another handle's memory
 };
 
 extern "C" const char* cf_last_error(void) { return g_err.c_str(); }
@@ -865,11 +866,25 @@ extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filter
 extern "C" void cf_spectrum_destroy(cf_spectrum_t h) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!h) return;
-  if (h->d) {
+  if (h->d && !h->view) {
     hipSetDevice(g.device);
     hipFree(h->d);
   }
   delete h;
+}
+
+extern "C" int cf_spectrum_view(cf_spectrum_t parent, unsigned index, cf_spectrum_t* out) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!parent || !out) return fail(CF_ERR_ARG, "NULL pointer");
+  if (index >= parent->count) return fail(CF_ERR_ARG, "index %u of a spectrum of %u filters", index, parent->count);
+  auto* h = new cf_spectrum_s;
+  h->n = parent->n;
+  h->count = 1;
+  h->plan = parent->plan;
+  h->d = (char*)parent->d + (size_t)index * cube_bytes(parent->n);
+  h->view = true;
+  *out = h;
+  return CF_OK;
 }
 
 extern "C" const void* cf_spectrum_data(cf_spectrum_t h) { return h ? h->d : nullptr; }
@@ -1220,20 +1235,27 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
       }
     }
     HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
-    // runs of consecutive instances with one edge, one spectrum and contiguous
-    // signals / outputs are recorded as one batched pass sequence
+    // runs of consecutive instances with one edge and contiguous signals /
+    // outputs, and either one spectrum or spectra that follow each other in
+    // memory (views of one multi-filter spectrum), are recorded as one batched
+    // pass sequence
     for (unsigned i = 0; i < count && rc == CF_OK;) {
       const unsigned n = inst[i].h->n;
       const size_t V = cube_bytes(n);
+      const bool splitp = is_split(inst[i].h->plan);
       unsigned run = 1;
-      while (i + run < count && inst[i + run].h == inst[i].h &&
+      const bool shared = i + 1 < count && inst[i + 1].h == inst[i].h;
+      while (i + run < count && !splitp && inst[i + run].h->n == n && inst[i + run].h->plan == inst[i].h->plan &&
+             (shared ? inst[i + run].h == inst[i].h
+                     : (const char*)inst[i + run].h->d == (const char*)inst[i].h->d + run * V) &&
              (const char*)inst[i + run].d_sig == (const char*)inst[i].d_sig + run * V &&
-             (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V && !is_split(inst[i].h->plan))
+             (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V)
         ++run;
+      const long long hstride = (run > 1 && !shared) ? (long long)cube_elems(n) : 0;
       const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
       std::vector<Pass> p;
-      conv_passes(*fns[i], inst[i].h->plan, n, run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, 0, scale,
-                  p);
+      conv_passes(*fns[i], inst[i].h->plan, n, run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, hstride,
+                  scale, p);
       rc = run_passes(p, n, run, g.capture, fns[i]->tw, nullptr);
       i += run;
     }

@@ -88,6 +88,12 @@ unsigned cf_spectrum_edge(cf_spectrum_t h);
 unsigned cf_spectrum_count(cf_spectrum_t h);
 /* the plan the spectrum was laid out for (its layout is internal to the plan) */
 int cf_spectrum_plan(cf_spectrum_t h);
+/* A handle of count 1 for filter `index` of `parent`, sharing its memory (no
+ * copy); valid while the parent lives, destroyed with cf_spectrum_destroy.
+ * Instances of a graph batch whose spectra are consecutive views of one
+ * parent (and whose signals and outputs are consecutive) run as one batched
+ * pass sequence. */
+int cf_spectrum_view(cf_spectrum_t parent, unsigned index, cf_spectrum_t *out);
 
 /* ---- device-resident convolution --------------------------------------------
  * how_many cubes d_sig -> d_out (may be the same buffer).  With a spectrum of

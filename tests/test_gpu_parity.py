@@ -357,6 +357,56 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode):
     b.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_spectrum_views_grouped(gpu, oracle, mode):
+    """Instances whose spectra are consecutive views of one multi-filter
+    spectrum and whose signals / outputs are consecutive slices: the graph
+    batch records each such run as one batched pass sequence (own filter per
+    cube); the order interleaves edges, so runs break and resume."""
+    import torch
+    api = gpu.api
+    counts = {32: 3, 64: 2, 128: 2}
+    print("case batch spectrum views mode=%d counts=%s" % (mode, counts), flush=True)
+    host, dev, specs = {}, {}, {}
+    for n, c in counts.items():
+        P = n ** 3
+        s = [oracle.uniform(P, 81000 + 31 * n + k) for k in range(c)]
+        f = [oracle.uniform(P, 82000 + 31 * n + k) for k in range(c)]
+        host[n] = (s, f)
+        dsig = _t(np.concatenate(s))
+        dev[n] = (dsig, torch.zeros_like(dsig))
+        parent = gpu.Spectrum(n, _t(np.concatenate(f)), count=c)
+        specs[n] = [parent.view(k) for k in range(c)]
+        del parent  # the views keep it alive
+    order = [(32, 0), (32, 1), (64, 0), (64, 1), (32, 2), (128, 0), (128, 1)]
+    inst = []
+    for n, k in order:
+        P = n ** 3
+        ds, do = dev[n]
+        inst.append((ds[k * P:(k + 1) * P], do[k * P:(k + 1) * P], specs[n][k]))
+    b = api.Batch(inst, mode=mode)
+    for rep in range(2):
+        for _, do in dev.values():
+            do.zero_()
+        b.launch()
+        b.check()
+        for n, c in counts.items():
+            got = _np(dev[n][1]).reshape(c, -1)
+            s, f = host[n]
+            for k in range(c):
+                ref = oracle.conv3d_f64(n, s[k], f[k], 1.0 / n ** 3)
+                assert oracle.rel_l2(got[k], ref) <= REL_L2_FP32, (n, k, rep)
+    b.close()
+    # a view used directly equals its filter's own spectrum
+    n = 32
+    s, f = host[n]
+    out = torch.zeros(n ** 3, dtype=torch.complex64, device="cuda")
+    gpu.conv3d_device(specs[n][2], _t(s[2]), out)
+    assert oracle.rel_l2(_np(out), oracle.conv3d_f64(n, s[2], f[2], 1.0 / n ** 3)) <= REL_L2_FP32
+    with pytest.raises(api.ConvError):
+        specs[n][0]._parent.view(counts[n])
+
+
 def test_batch_shared_spectrum_in_place(gpu, oracle):
     n, k = 64, 4
     print("case batch shared spectrum in place", flush=True)
