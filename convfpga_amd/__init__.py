@@ -6,5 +6,5 @@ this package only builds it (convfpga_amd.build) and binds it from Python
 """
 from . import api  # noqa: F401
 from .api import (CODE_OBJECT, LIB_PATH, SUPPORTED_SIZES, ConvError, Spectrum, conv3d_device,  # noqa: F401
-                  conv3d_device_timed, fft3d_device, fpga_final, fpga_initialize, fpgaf_complex_malloc,
+                  conv3d_device_timed, conv3d_host, fft3d_device, fpga_final, fpga_initialize, fpgaf_complex_malloc,
                   fpgaf_conv3D, fpgaf_conv3D_svm, fpgaf_conv3D_svm_batch, rel_l2_device)

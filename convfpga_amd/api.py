@@ -38,7 +38,7 @@ SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpgaf_conv3D",
                   "fpgaf_conv3D_svm", "fpgaf_conv3D_svm_batch")
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
-               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device",
+               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device", "cf_conv3d_host",
                "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
@@ -64,6 +64,10 @@ class cf_pass_times_t(ctypes.Structure):
     def as_list(self):
         return [{"name": self.name[i].value.decode(), "ms": float(self.ms[i]), "bytes": float(self.bytes[i])}
                 for i in range(self.npass)]
+
+
+class cf_host_times_t(ctypes.Structure):
+    _fields_ = [("pass_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
 
 
 class cf_instance_t(ctypes.Structure):
@@ -101,6 +105,7 @@ def lib():
             "cf_spectrum_count": (u, [vp]),
             "cf_spectrum_view": (i, [vp, u, ctypes.POINTER(vp)]),
             "cf_conv3d_device": (i, [vp, vp, vp, u, i, i, vp]),
+            "cf_conv3d_host": (i, [vp, vp, vp, u, i, i, ctypes.POINTER(cf_host_times_t)]),
             "cf_conv3d_device_timed": (i, [vp, vp, vp, u, i, i, vp, ctypes.POINTER(cf_pass_times_t)]),
             "cf_fft3d_device": (i, [u, vp, vp, u, i, vp]),
             "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
@@ -258,6 +263,17 @@ class Spectrum:
 def conv3d_device(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
     _check(lib().cf_conv3d_device(spec.handle, _dev(sig), _dev(out), how_many, scale, precision, _stream(stream)),
            "cf_conv3d_device")
+
+
+def conv3d_host(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32):
+    """Host-resident batch through the overlapped upload / passes / download
+    pipeline (cf_conv3d_host); sig and out are numpy complex64 arrays of
+    how_many cubes (may be the same array).  Returns (pass_ms, wall_ms)."""
+    P = spec.N ** 3 * how_many
+    t = cf_host_times_t()
+    _check(lib().cf_conv3d_host(spec.handle, _host(sig, P), _host(out, P), how_many, scale, precision,
+                                ctypes.byref(t)), "cf_conv3d_host")
+    return t.pass_ms, t.wall_ms
 
 
 def conv3d_device_timed(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):

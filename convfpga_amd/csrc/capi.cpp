@@ -73,7 +73,7 @@ fpga_t fpgaf_conv3D_svm(const unsigned N, const float2* sig, const float2* filte
 
 fpga_t fpgaf_conv3D_svm_batch(const unsigned N, const float2* sig, const float2* filter, float2* out,
                               const unsigned how_many) {
-  return run(N, sig, filter, out, how_many, CFE_MODE_SVM);
+  return run(N, sig, filter, out, how_many, CFE_MODE_SVM_BATCH);
 }
 
 }  // extern "C"

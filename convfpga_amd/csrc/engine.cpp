@@ -19,12 +19,15 @@
 #include <cctype>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batch_desc.h"
@@ -140,9 +143,12 @@ std::string lower(const std::string& s) {
   return r;
 }
 
+void pipe_release();
+
 void release_all() {
   if (g.ready || g.stream) hipSetDevice(g.device);
   if (g.stream) hipStreamSynchronize(g.stream);
+  pipe_release();
   for (auto& kv : g.sizes)
     if (kv.second.tw) hipFree(kv.second.tw);
   g.sizes.clear();
@@ -450,8 +456,18 @@ int ensure_pinned(void** p, size_t* have, size_t need) {
 // of the other, and the host copies use a few threads.
 constexpr size_t kStageChunk = size_t(8) << 20;
 
+// threads per host copy: 4, or $CONVFPGA_COPY_THREADS (1..32)
+int copy_threads() {
+  static const int t = [] {
+    const char* e = getenv("CONVFPGA_COPY_THREADS");
+    const int v = e ? atoi(e) : 0;
+    return v >= 1 && v <= 32 ? v : 4;
+  }();
+  return t;
+}
+
 void host_copy(void* dst, const void* src, size_t n) {
-  constexpr int kThreads = 4;
+  const int kThreads = copy_threads();
   if (n < (size_t(1) << 20)) {
     memcpy(dst, src, n);
     return;
@@ -653,6 +669,280 @@ int choose_plan(unsigned n, SizeFns* f, int* plan) {
   return CF_OK;
 }
 
+// ---- host-resident batches: transfers overlapped with the passes -------------
+// Cubes move in groups of whole cubes (about kPipeGroupBytes each) through
+// kPipeSlots device slots.  The calling thread copies group k from pageable
+// memory into two pinned chunk buffers and uploads it on s_in, then queues the
+// group's passes (in place in its slot) on the compute stream; a second host
+// thread downloads finished groups on s_out through two more pinned chunk
+// buffers and copies them out.  Upload of group k+1, the passes of group k and
+// the download of group k-1 therefore run at the same time, the idiomatic
+// analogue of the reference's pairwise pipelining of consecutive instances
+// (convfpga_svm_batch.c:241-662).
+//
+// Ordering, all on the device through events:
+//   slot s is refilled (group k+S) only after the download of group k has
+//   read it (ev_freed[s], recorded on s_out);
+//   the passes of group k wait for its upload (ev_loaded[s], on s_in);
+//   the download of group k waits for its passes (ev_done[s], compute stream).
+// Each event is re-recorded only after every wait on its previous record has
+// been queued: the two host threads hand over through `computed` (groups
+// whose passes are queued) and `drained` (groups whose downloads are queued).
+constexpr int kPipeSlots = 3;
+constexpr size_t kPipeGroupBytes = size_t(16) << 20;
+
+struct PipeState {
+  hipStream_t s_in = nullptr, s_out = nullptr;
+  void* stage_in[2] = {nullptr, nullptr};
+  void* stage_out[2] = {nullptr, nullptr};
+  hipEvent_t ev_in[2] = {}, ev_out[2] = {};
+  hipEvent_t ev_loaded[kPipeSlots] = {}, ev_done[kPipeSlots] = {}, ev_freed[kPipeSlots] = {};
+  void* slots = nullptr;
+  size_t slots_bytes = 0;
+};
+PipeState gp;
+
+void pipe_release() {
+  if (gp.s_in) hipStreamSynchronize(gp.s_in);
+  if (gp.s_out) hipStreamSynchronize(gp.s_out);
+  for (int b = 0; b < 2; ++b) {
+    if (gp.stage_in[b]) hipHostFree(gp.stage_in[b]);
+    if (gp.stage_out[b]) hipHostFree(gp.stage_out[b]);
+    if (gp.ev_in[b]) hipEventDestroy(gp.ev_in[b]);
+    if (gp.ev_out[b]) hipEventDestroy(gp.ev_out[b]);
+  }
+  for (int i = 0; i < kPipeSlots; ++i) {
+    if (gp.ev_loaded[i]) hipEventDestroy(gp.ev_loaded[i]);
+    if (gp.ev_done[i]) hipEventDestroy(gp.ev_done[i]);
+    if (gp.ev_freed[i]) hipEventDestroy(gp.ev_freed[i]);
+  }
+  if (gp.slots) hipFree(gp.slots);
+  if (gp.s_in) hipStreamDestroy(gp.s_in);
+  if (gp.s_out) hipStreamDestroy(gp.s_out);
+  gp = PipeState();
+}
+
+int pipe_ready(size_t slot_bytes) {
+  if (!gp.s_in) HIPCHK(hipStreamCreateWithFlags(&gp.s_in, hipStreamNonBlocking));
+  if (!gp.s_out) HIPCHK(hipStreamCreateWithFlags(&gp.s_out, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b) {
+    if (!gp.stage_in[b]) HIPCHK(hipHostMalloc(&gp.stage_in[b], kStageChunk, hipHostMallocDefault));
+    if (!gp.stage_out[b]) HIPCHK(hipHostMalloc(&gp.stage_out[b], kStageChunk, hipHostMallocDefault));
+    if (!gp.ev_in[b]) HIPCHK(hipEventCreateWithFlags(&gp.ev_in[b], hipEventDisableTiming));
+    if (!gp.ev_out[b]) HIPCHK(hipEventCreateWithFlags(&gp.ev_out[b], hipEventDisableTiming));
+  }
+  for (int i = 0; i < kPipeSlots; ++i) {
+    if (!gp.ev_loaded[i]) HIPCHK(hipEventCreateWithFlags(&gp.ev_loaded[i], hipEventDisableTiming));
+    if (!gp.ev_done[i]) HIPCHK(hipEventCreateWithFlags(&gp.ev_done[i], hipEventDisableTiming));
+    if (!gp.ev_freed[i]) HIPCHK(hipEventCreateWithFlags(&gp.ev_freed[i], hipEventDisableTiming));
+  }
+  const size_t need = slot_bytes * kPipeSlots;
+  if (gp.slots_bytes < need) {
+    if (gp.slots) {
+      HIPCHK(hipDeviceSynchronize());
+      hipFree(gp.slots);
+    }
+    gp.slots = nullptr;
+    gp.slots_bytes = 0;
+    const hipError_t e = hipMalloc(&gp.slots, need);
+    if (e != hipSuccess) return fail(CF_ERR_NOMEM, "hipMalloc(%zu) for the batch slots: %s", need, hipGetErrorString(e));
+    gp.slots_bytes = need;
+  }
+  return CF_OK;
+}
+
+struct PipeJob {
+  const SizeFns* f;
+  int plan, precision;
+  unsigned n, how_many, group;  // cubes per group (the last one may hold fewer)
+  const void* H;                // spectrum; hstride 0 (shared) or N^3 (one per cube)
+  long long hstride;
+  float scale;
+  const char* sig;
+  char* out;
+  hipStream_t s;  // compute stream
+};
+
+struct PipeSync {
+  std::mutex mu;
+  std::condition_variable cv;
+  unsigned computed = 0, drained = 0;
+  bool abort = false;
+  int rc = CF_OK;
+  std::string err;
+};
+
+// download side: runs on its own host thread
+void pipe_drain(const PipeJob& j, PipeSync& ps) {
+  auto stop = [&](int rc) {
+    std::lock_guard<std::mutex> lk(ps.mu);
+    if (ps.rc == CF_OK) {
+      ps.rc = rc;
+      ps.err = g_err;  // thread-local in this thread
+    }
+    ps.abort = true;
+    ps.cv.notify_all();
+  };
+  if (hipSetDevice(g.device) != hipSuccess) return stop(fail(CF_ERR_RUNTIME, "hipSetDevice failed"));
+  const size_t V = cube_bytes(j.n);
+  const unsigned ngroups = (j.how_many + j.group - 1) / j.group;
+  struct Chunk {
+    size_t dst_off, bytes;
+    int b;
+  };
+  Chunk pend{0, 0, -1};  // downloaded chunk not yet copied out
+  unsigned nchunk = 0;
+  auto finish = [&](const Chunk& c) -> int {
+    HIPCHK(hipEventSynchronize(gp.ev_out[c.b]));
+    host_copy(j.out + c.dst_off, gp.stage_out[c.b], c.bytes);
+    return CF_OK;
+  };
+  for (unsigned k = 0; k < ngroups; ++k) {
+    {
+      std::unique_lock<std::mutex> lk(ps.mu);
+      ps.cv.wait(lk, [&] { return ps.abort || ps.computed > k; });
+      if (ps.abort) return;
+    }
+    const int sl = (int)(k % kPipeSlots);
+    const size_t k0 = (size_t)k * j.group;
+    const size_t bytes = std::min<size_t>(j.group, j.how_many - k0) * V;
+    const char* src = (const char*)gp.slots + sl * (size_t)j.group * V;
+    if (hipStreamWaitEvent(gp.s_out, gp.ev_done[sl], 0) != hipSuccess)
+      return stop(fail(CF_ERR_RUNTIME, "hipStreamWaitEvent (download)"));
+    for (size_t off = 0; off < bytes; off += kStageChunk) {
+      const Chunk c{k0 * V + off, std::min(kStageChunk, bytes - off), (int)(nchunk++ & 1)};
+      // buffer c.b last held the chunk before `pend`, already copied out
+      if (hipMemcpyAsync(gp.stage_out[c.b], src + off, c.bytes, hipMemcpyDeviceToHost, gp.s_out) != hipSuccess ||
+          hipEventRecord(gp.ev_out[c.b], gp.s_out) != hipSuccess)
+        return stop(fail(CF_ERR_RUNTIME, "download of group %u failed", k));
+      if (off + kStageChunk >= bytes) {  // slot sl fully read once this is done
+        if (hipEventRecord(gp.ev_freed[sl], gp.s_out) != hipSuccess)
+          return stop(fail(CF_ERR_RUNTIME, "hipEventRecord (slot freed)"));
+        std::lock_guard<std::mutex> lk(ps.mu);
+        ps.drained = k + 1;
+        ps.cv.notify_all();
+      }
+      if (pend.b >= 0) {
+        const int rc = finish(pend);
+        if (rc != CF_OK) return stop(rc);
+      }
+      pend = c;
+    }
+  }
+  if (pend.b >= 0) {
+    const int rc = finish(pend);
+    if (rc != CF_OK) return stop(rc);
+  }
+}
+
+// upload side and passes: the calling thread.  Returns the summed span of the
+// groups' passes (host clock of the compute events is not needed: spans come
+// from events recorded around each group).
+int pipe_feed(const PipeJob& j, PipeSync& ps, std::vector<hipEvent_t>& tev) {
+  const size_t V = cube_bytes(j.n);
+  const unsigned ngroups = (j.how_many + j.group - 1) / j.group;
+  const size_t slot_bytes = (size_t)j.group * V;
+  unsigned nchunk = 0;
+  bool used[2] = {false, false};
+  auto stop = [&](int rc) {
+    std::lock_guard<std::mutex> lk(ps.mu);
+    if (ps.rc == CF_OK) {
+      ps.rc = rc;
+      ps.err = g_err;
+    }
+    ps.abort = true;
+    ps.cv.notify_all();
+    return rc;
+  };
+  for (unsigned k = 0; k < ngroups; ++k) {
+    const int sl = (int)(k % kPipeSlots);
+    const size_t k0 = (size_t)k * j.group;
+    const unsigned nb = (unsigned)std::min<size_t>(j.group, j.how_many - k0);
+    const size_t bytes = nb * V;
+    char* slot = (char*)gp.slots + sl * slot_bytes;
+    if (k >= (unsigned)kPipeSlots) {  // the download of group k-S has to be queued before its event is waited on
+      std::unique_lock<std::mutex> lk(ps.mu);
+      ps.cv.wait(lk, [&] { return ps.abort || ps.drained > k - kPipeSlots; });
+      if (ps.abort) return ps.rc;
+      lk.unlock();
+      if (hipStreamWaitEvent(gp.s_in, gp.ev_freed[sl], 0) != hipSuccess)
+        return stop(fail(CF_ERR_RUNTIME, "hipStreamWaitEvent (slot reuse)"));
+    }
+    for (size_t off = 0; off < bytes; off += kStageChunk) {
+      const int b = (int)(nchunk++ & 1);
+      const size_t cb = std::min(kStageChunk, bytes - off);
+      if (used[b] && hipEventSynchronize(gp.ev_in[b]) != hipSuccess)
+        return stop(fail(CF_ERR_RUNTIME, "upload of group %u failed", k));
+      host_copy(gp.stage_in[b], j.sig + k0 * V + off, cb);
+      if (hipMemcpyAsync(slot + off, gp.stage_in[b], cb, hipMemcpyHostToDevice, gp.s_in) != hipSuccess ||
+          hipEventRecord(gp.ev_in[b], gp.s_in) != hipSuccess)
+        return stop(fail(CF_ERR_RUNTIME, "upload of group %u failed", k));
+      used[b] = true;
+    }
+    if (hipEventRecord(gp.ev_loaded[sl], gp.s_in) != hipSuccess ||
+        hipStreamWaitEvent(j.s, gp.ev_loaded[sl], 0) != hipSuccess)
+      return stop(fail(CF_ERR_RUNTIME, "upload event of group %u", k));
+    std::vector<Pass> p;
+    const void* H = (const char*)j.H + (size_t)k0 * j.hstride * 8;
+    void* work = nullptr;
+    if (is_split(j.plan)) {
+      const int rc = acquire_work(slot_bytes, j.s, &work);
+      if (rc != CF_OK) return stop(rc);
+    }
+    conv_passes(*j.f, j.plan, j.n, nb, slot, slot, work, H, j.hstride, j.scale, p, j.precision);
+    if (hipEventRecord(tev[2 * k], j.s) != hipSuccess) return stop(fail(CF_ERR_RUNTIME, "hipEventRecord"));
+    int rc = run_passes(p, j.n, nb, j.s, j.f->tw, nullptr);
+    if (work) release_work(j.s);
+    if (rc != CF_OK) return stop(rc);
+    if (hipEventRecord(tev[2 * k + 1], j.s) != hipSuccess || hipEventRecord(gp.ev_done[sl], j.s) != hipSuccess)
+      return stop(fail(CF_ERR_RUNTIME, "hipEventRecord (passes done)"));
+    std::lock_guard<std::mutex> lk(ps.mu);
+    ps.computed = k + 1;
+    ps.cv.notify_all();
+  }
+  return CF_OK;
+}
+
+// Runs a host-resident batch through the pipeline; blocks until the last
+// output byte is in `out`.  pass_ms: summed span of the groups' passes;
+// wall_ms: first upload to last copy-out, host clock.
+int pipe_run(PipeJob j, double* pass_ms, double* wall_ms) {
+  const size_t V = cube_bytes(j.n);
+  j.group = (unsigned)std::max<size_t>(1, std::min<size_t>(j.how_many, kPipeGroupBytes / V));
+  const unsigned ngroups = (j.how_many + j.group - 1) / j.group;
+  CFCHK(pipe_ready((size_t)j.group * V));
+  std::vector<hipEvent_t> tev(2 * (size_t)ngroups, nullptr);
+  struct Events {
+    std::vector<hipEvent_t>& v;
+    ~Events() {
+      for (auto e : v)
+        if (e) hipEventDestroy(e);
+    }
+  } hold{tev};
+  for (auto& e : tev) HIPCHK(hipEventCreate(&e));
+  // earlier work on the compute stream (e.g. the filter transform) stays first
+  const auto t0 = std::chrono::steady_clock::now();
+  PipeSync ps;
+  std::thread drain(pipe_drain, std::cref(j), std::ref(ps));
+  pipe_feed(j, ps, tev);
+  drain.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  const hipError_t e1 = hipStreamSynchronize(j.s), e2 = hipStreamSynchronize(gp.s_in),
+                   e3 = hipStreamSynchronize(gp.s_out);
+  if (ps.rc != CF_OK) {
+    g_err = ps.err;
+    return ps.rc;
+  }
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(CF_ERR_RUNTIME, "host batch pipeline: %s",
+                hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+  double sum = 0.0;
+  for (unsigned k = 0; k < ngroups; ++k) sum += ms_between(tev[2 * k], tev[2 * k + 1]);
+  if (pass_ms) *pass_ms = sum;
+  if (wall_ms) *wall_ms = host_ms(t0, t1);
+  return CF_OK;
+}
+
 }  // namespace
 
 // =============================================================================
@@ -719,9 +1009,10 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
                              cfe_times_t* t) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!sig || !filter || !out || how_many == 0 || !t) return fail(CF_ERR_ARG, "NULL pointer or zero count");
-  if (mode != CFE_MODE_COPY && mode != CFE_MODE_SVM) return fail(CF_ERR_ARG, "unknown mode %d", mode);
+  if (mode != CFE_MODE_COPY && mode != CFE_MODE_SVM && mode != CFE_MODE_SVM_BATCH)
+    return fail(CF_ERR_ARG, "unknown mode %d", mode);
   CFCHK(require_ready());
-  if (mode == CFE_MODE_SVM && !g.svm) return fail(CF_ERR_STATE, "fpga_initialize() was called with use_svm = false");
+  if (mode != CFE_MODE_COPY && !g.svm) return fail(CF_ERR_STATE, "fpga_initialize() was called with use_svm = false");
   SizeFns* f = nullptr;
   CFCHK(size_res(n, &f));
   int plan = 0;
@@ -732,10 +1023,31 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   hipEvent_t* ev = g.ev;
   hipStream_t s = g.stream;
   CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
+  std::vector<Pass> filt_p, conv_p;
+  if (mode == CFE_MODE_SVM_BATCH) {
+    // filter upload and transform, then the cubes stream through the
+    // overlapped pipeline; times as the reference's batch call reports them
+    // (convfpga_svm_batch.c:664-665): summed pass spans and the wall span
+    void* fwork = nullptr;
+    if (is_split(plan)) CFCHK(acquire_work(V, s, &fwork));
+    CFCHK(staged_h2d(g.d_filt, filter, V, s));
+    spectrum_passes(*f, plan, n, 1, g.d_filt, g.d_filt, fwork, filt_p);
+    const size_t nf = filt_p.size();
+    CFCHK(ensure_events());
+    const int rc = run_passes(filt_p, n, 1, s, f->tw, ev);  // ev[0 .. nf]
+    if (fwork) release_work(s);
+    CFCHK(rc);
+    PipeJob j{f, plan, CF_PREC_FP32, n, how_many, 1, g.d_filt, 0, 1.0f, (const char*)sig, (char*)out, s};
+    double pass_ms = 0.0, wall_ms = 0.0;
+    CFCHK(pipe_run(j, &pass_ms, &wall_ms));
+    t->filter_exec_t = ms_between(ev[0], ev[nf]);
+    t->sig_exec_t = pass_ms;
+    t->siginv_exec_t = wall_ms;
+    return CF_OK;
+  }
   CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
   void* work = nullptr;
   if (is_split(plan)) CFCHK(acquire_work(SV, s, &work));
-  std::vector<Pass> filt_p, conv_p;
 
   if (mode == CFE_MODE_COPY) {
     // Blocking phases as in the reference (convfpga.c:75-375); each host
@@ -925,9 +1237,7 @@ extern "C" int cf_autotune(unsigned n, float* ms) {
 extern "C" unsigned cf_spectrum_edge(cf_spectrum_t h) { return h ? h->n : 0; }
 extern "C" unsigned cf_spectrum_count(cf_spectrum_t h) { return h ? h->count : 0; }
 
-static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many, int scale_mode,
-                            int precision, void* stream, cf_pass_times_t* times) {
-  if (!h || !d_sig || !d_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+static int check_conv_args(cf_spectrum_t h, unsigned how_many, int scale_mode, int precision) {
   if (h->count != 1 && h->count != how_many)
     return fail(CF_ERR_ARG, "spectrum holds %u filters; need 1 or how_many (%u)", h->count, how_many);
   if (scale_mode != CF_SCALE_NONE && scale_mode != CF_SCALE_INV_N3) return fail(CF_ERR_ARG, "bad scale_mode");
@@ -939,6 +1249,13 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
                 "split-FP16 precision runs N = 256 with a spectrum of a four-step plan with middle tiles of 2 x "
                 "(split3_g22, g42, g24m2, g28m2); this spectrum is N = %u, plan %d",
                 h->n, h->plan);
+  return CF_OK;
+}
+
+static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many, int scale_mode,
+                            int precision, void* stream, cf_pass_times_t* times) {
+  if (!h || !d_sig || !d_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  CFCHK(check_conv_args(h, how_many, scale_mode, precision));
   CFCHK(require_ready());
   SizeFns* f = nullptr;
   CFCHK(size_res(h->n, &f));
@@ -992,6 +1309,27 @@ extern "C" int cf_conv3d_device(cf_spectrum_t h, const void* d_sig, void* d_out,
                                 int precision, void* stream) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   return conv_device_impl(h, d_sig, d_out, how_many, scale_mode, precision, stream, nullptr);
+}
+
+extern "C" int cf_conv3d_host(cf_spectrum_t h, const void* h_sig, void* h_out, unsigned how_many, int scale_mode,
+                              int precision, cf_host_times_t* times) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!h || !h_sig || !h_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  CFCHK(check_conv_args(h, how_many, scale_mode, precision));
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(h->n, &f));
+  const unsigned n = h->n;
+  const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+  const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
+  PipeJob j{f, h->plan, precision, n, how_many, 1, h->d, hstride, scale, (const char*)h_sig, (char*)h_out, g.stream};
+  double pass_ms = 0.0, wall_ms = 0.0;
+  CFCHK(pipe_run(j, &pass_ms, &wall_ms));
+  if (times) {
+    times->pass_ms = pass_ms;
+    times->wall_ms = wall_ms;
+  }
+  return CF_OK;
 }
 
 extern "C" int cf_conv3d_device_timed(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many,

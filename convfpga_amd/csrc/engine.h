@@ -14,7 +14,9 @@ typedef struct {
 } cfe_times_t;
 
 #define CFE_MODE_COPY 0 /* explicit host<->device copies (fpgaf_conv3D)            */
-#define CFE_MODE_SVM 1  /* kernels read/write pinned host memory (fpgaf_conv3D_svm*) */
+#define CFE_MODE_SVM 1  /* kernels read/write pinned host memory (fpgaf_conv3D_svm)  */
+#define CFE_MODE_SVM_BATCH 2 /* cubes streamed through overlapped transfers and passes
+                                (fpgaf_conv3D_svm_batch)                                */
 
 int cfe_initialize(const char *platform_name, const char *path, int use_svm);
 void cfe_final(void);

@@ -101,6 +101,23 @@ int cf_spectrum_view(cf_spectrum_t parent, unsigned index, cf_spectrum_t *out);
 int cf_conv3d_device(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned how_many, int scale_mode,
                      int precision, void *stream);
 
+/* ---- host-resident batches --------------------------------------------------
+ * how_many cubes h_sig -> h_out in ordinary (pageable) host memory; the two
+ * may be the same buffer.  The cubes stream through the device in groups of
+ * whole cubes so that the upload of one group, the passes of the group before
+ * it and the download of the one before that run at the same time (three
+ * streams of the library, pinned chunk buffers, three device slots): the
+ * overlapped counterpart of the reference's pairwise batch pipeline
+ * (host/src/convfpga_svm_batch.c:241-662), with any spectrum of
+ * cf_conv3d_device (shared, or one filter per cube), scale mode and
+ * precision.  Blocking.  times (nullable): summed device time of the groups'
+ * passes and the wall time of the whole call, in milliseconds. */
+typedef struct {
+  double pass_ms, wall_ms;
+} cf_host_times_t;
+int cf_conv3d_host(cf_spectrum_t h, const void *h_sig, void *h_out, unsigned how_many, int scale_mode,
+                   int precision, cf_host_times_t *times);
+
 /* Per-pass device times (milliseconds) of one cf_conv3d_device call, measured
  * with HIP events on the launch stream; blocks until the call has finished. */
 #define CF_MAX_PASSES 8

@@ -29,6 +29,9 @@ int main() {
   cf_spectrum_t h = nullptr;
   chk(cf_spectrum_create(64, 1, a, 0, nullptr, &h) != 0, "spectrum not init");
   chk(cf_conv3d_device(nullptr, a, c, 1, 0, 0, nullptr) != 0, "conv null handle");
+  cf_host_times_t ht;
+  chk(cf_conv3d_host(nullptr, a, c, 1, 0, 0, &ht) != 0, "host conv null handle");
+  chk(cf_spectrum_view(nullptr, 0, &h) != 0, "view null parent");
   cf_batch_t bt = nullptr;
   chk(cf_batch_create(nullptr, 1, 0, 0, &bt) != 0, "batch null");
   chk(strlen(cf_last_error()) > 0, "last error");

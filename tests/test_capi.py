@@ -98,6 +98,8 @@ def test_ext_api_reports_state_errors():
     assert rc == api.CF_ERR_STATE and "fpga_initialize" in api.last_error()
     assert L.cf_spectrum_create(8, 0, buf.ctypes.data, 0, None, ctypes.byref(h)) == api.CF_ERR_ARG
     assert L.cf_conv3d_device(None, None, None, 1, 0, 0, None) == api.CF_ERR_ARG
+    assert L.cf_conv3d_host(None, buf.ctypes.data, buf.ctypes.data, 1, 0, 0, None) == api.CF_ERR_ARG
+    assert L.cf_spectrum_view(None, 0, ctypes.byref(h)) == api.CF_ERR_ARG
     r = ctypes.c_double()
     assert L.cf_rel_l2_device(None, None, 0, 10, None, ctypes.byref(r)) == api.CF_ERR_ARG
     assert L.cf_fft3d_device(8, buf.ctypes.data, buf.ctypes.data, 1, 0, None) == api.CF_ERR_STATE

@@ -66,6 +66,24 @@ def test_dropin_svm_batch_distinct_instances(gpu, oracle, how_many):
         assert oracle.rel_l2(np.ascontiguousarray(out[s]), ref) <= REL_L2_FP32
 
 
+def test_dropin_svm_batch_streams_groups(gpu, oracle):
+    """fpgaf_conv3D_svm_batch at a size where every cube is its own group of
+    the overlapped upload / passes / download pipeline and the device slots
+    are reused (5 cubes, 3 slots); odd count."""
+    n, how_many = 128, 5
+    print("case fpgaf_conv3D_svm_batch N=%d how_many=%d (pipelined)" % (n, how_many), flush=True)
+    sig = oracle.uniform(how_many * n ** 3, 5100)
+    filt = oracle.uniform(n ** 3, 6100)
+    out = np.zeros_like(sig)
+    r = gpu.fpgaf_conv3D_svm_batch(n, sig, filt, out, how_many)
+    assert r.valid and r.sig_exec_t > 0 and r.siginv_exec_t >= r.sig_exec_t
+    assert r.filter_pcie_wr_t == 0 and r.sig_pcie_wr_t == 0 and r.sig_pcie_rd_t == 0
+    for b in range(how_many):
+        s = slice(b * n ** 3, (b + 1) * n ** 3)
+        ref = oracle.conv3d_f64(n, np.ascontiguousarray(sig[s]), filt)
+        assert oracle.rel_l2(np.ascontiguousarray(out[s]), ref) <= REL_L2_FP32, b
+
+
 def test_dropin_reference_generator(gpu, oracle):
     """The reference's own inputs (rand()/RAND_MAX, helper.cpp:148-172) at its default N=64."""
     n = 64
@@ -405,6 +423,54 @@ def test_batch_spectrum_views_grouped(gpu, oracle, mode):
     assert oracle.rel_l2(_np(out), oracle.conv3d_f64(n, s[2], f[2], 1.0 / n ** 3)) <= REL_L2_FP32
     with pytest.raises(api.ConvError):
         specs[n][0]._parent.view(counts[n])
+
+
+@pytest.mark.parametrize("n,how_many,own,in_place", [(64, 20, False, False), (64, 20, True, True),
+                                                     (32, 70, True, False), (128, 5, False, True)])
+def test_host_pipeline_matches_oracle(gpu, oracle, n, how_many, own, in_place):
+    """cf_conv3d_host: host-resident cubes through the overlapped pipeline.
+    Groups of 16 MiB of whole cubes: 64^3 x20 -> groups of 8, 8, 4 (ragged);
+    32^3 x70 -> 64 + 6; 128^3 x5 -> five groups of one over three slots."""
+    print("case cf_conv3d_host N=%d how_many=%d own=%s in_place=%s" % (n, how_many, own, in_place), flush=True)
+    P = n ** 3
+    sig = oracle.uniform(how_many * P, 90000 + n + how_many)
+    filt = oracle.uniform((how_many if own else 1) * P, 91000 + n)
+    spec = gpu.Spectrum(n, _t(filt), count=how_many if own else 1)
+    out = sig.copy() if in_place else np.zeros_like(sig)
+    pass_ms, wall_ms = gpu.conv3d_host(spec, out if in_place else sig, out, how_many)
+    assert 0 < pass_ms <= wall_ms
+    for b in range(how_many):
+        s = slice(b * P, (b + 1) * P)
+        f = np.ascontiguousarray(filt[s] if own else filt)
+        ref = oracle.conv3d_f64(n, np.ascontiguousarray(sig[s]), f, 1.0 / P)
+        assert oracle.rel_l2(np.ascontiguousarray(out[s]), ref) <= REL_L2_FP32, b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_host_pipeline_256_equals_device_path(gpu, oracle, precision):
+    """4 x 256^3 through cf_conv3d_host equal, bit for bit, the same
+    convolutions on device-resident data (same kernels, one cube per group,
+    sixteen 8 MiB chunks per transfer)."""
+    import torch
+    api = gpu.api
+    n, how_many = 256, 4
+    print("case cf_conv3d_host 256^3 x4 precision=%d" % precision, flush=True)
+    P = n ** 3
+    if precision == api.CF_PREC_SPLIT_FP16:
+        api.set_plan(256, api.CF_PLAN_SPLIT3_G24M2)
+    dsig = torch.empty(how_many * P, dtype=torch.complex64, device="cuda")
+    dfilt = torch.empty(how_many * P, dtype=torch.complex64, device="cuda")
+    api.fill_uniform(dsig, 2 * how_many * P, 4242)
+    api.fill_uniform(dfilt, 2 * how_many * P, 4343)
+    spec = gpu.Spectrum(n, dfilt, count=how_many)
+    api.set_plan(256, api.CF_PLAN_AUTO)
+    dout = torch.empty_like(dsig)
+    gpu.conv3d_device(spec, dsig, dout, how_many=how_many, precision=precision)
+    want = _np(dout)
+    hsig = _np(dsig)
+    hout = np.zeros_like(hsig)
+    gpu.conv3d_host(spec, hsig, hout, how_many, precision=precision)
+    assert np.array_equal(hout.view(np.uint32), want.view(np.uint32))
 
 
 def test_batch_shared_spectrum_in_place(gpu, oracle):
