@@ -194,6 +194,7 @@ def run_single(a, torch, C, api):
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
+        line["host_batch"] = host_batch_leg(torch, C, n, spec, 4)
     return line, (n, host_out, seed_sig, seed_filt)
 
 
@@ -242,6 +243,24 @@ def dropin_host_leg(C, n, sig, filt, reps=3):
         res[name] = {"wall_ms_best": round(1e3 * walls[k], 3), "conv_per_s": 1.0 / walls[k],
                      "fpga_t_ms": {f: round(v, 4) for f, v in fields[k].items() if f != "valid"}}
     return res
+
+
+def host_batch_leg(torch, C, n, spec, count, reps=3):
+    """`count` cubes in pageable host memory through cf_conv3d_host (uploads,
+    passes and downloads overlapped); compare ms_per_cube with the sequential
+    phases of fpgaf_conv3D in dropin_host_calls."""
+    import numpy as np
+    P = n ** 3
+    d = torch.empty(count * P, dtype=torch.complex64, device="cuda")
+    C.api.fill_uniform(d, 2 * count * P, 77)
+    hs = d.cpu().numpy()
+    ho = np.empty_like(hs)
+    walls = []
+    for _ in range(reps):
+        walls.append(C.conv3d_host(spec, hs, ho, count)[1])
+    w = min(walls)
+    return {"cubes": count, "pipelined_wall_ms": round(w, 3), "ms_per_cube": round(w / count, 3),
+            "conv_per_s": 1e3 * count / w, "host_bus_GBps_both_ways": 2 * 8.0 * P * count / (w * 1e-3) / 1e9}
 
 
 def run_batch(a, torch, C, api):
