@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--host-calls", action="store_true",
                     help="also time the drop-in calls on host buffers (PCIe included; not the metric)")
+    ap.add_argument("--power", action="store_true",
+                    help="also sample GPU power (amdsmi) over a separate 2 s loop of the same convolution")
     return ap.parse_args()
 
 
@@ -195,7 +197,40 @@ def run_single(a, torch, C, api):
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
         line["host_batch"] = host_batch_leg(torch, C, n, spec, 4)
+    if a.power:
+        line["power"] = power_leg(torch, C, spec, sig, out)
     return line, (n, host_out, seed_sig, seed_filt)
+
+
+def power_leg(torch, C, spec, sig, out, seconds=2.0, idle_s=0.5):
+    """Board power while the convolution runs back to back for `seconds`
+    (outside the timed region), the idle power before it, and convolutions
+    per joule from the GPU's energy counter (sampled power if it has none)."""
+    from convfpga_amd.power import PowerMonitor, PowerUnavailable, SmiReader, torch_device_bdf
+    try:
+        rd = SmiReader(torch_device_bdf(0))
+    except PowerUnavailable as e:
+        return {"unavailable": str(e)}
+    try:
+        torch.cuda.synchronize()
+        idle = PowerMonitor(rd, 0.01).start()
+        time.sleep(idle_s)
+        idle_s = idle.stop()
+        mon = PowerMonitor(rd, 0.01).start()
+        count, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(100):
+                C.conv3d_device(spec, sig, out)
+            torch.cuda.synchronize()
+            count += 100
+        s = mon.stop()
+    finally:
+        rd.close()
+    joules = s.get("energy_J") or ((s["mean_W"] or 0.0) * s["seconds"])
+    res = {"gpu": rd.bdf, "idle_W": idle_s["mean_W"], "busy_mean_W": s["mean_W"], "busy_max_W": s["max_W"],
+           "energy_counter_mean_W": s.get("energy_mean_W"), "samples": s["samples"], "convs": count,
+           "conv_per_s": count / s["seconds"], "conv_per_J": count / joules if joules else None}
+    return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()}
 
 
 def own_filter_leg(torch, C, n, sig, filt, out, reps=20):

@@ -14,6 +14,10 @@
 //   -s, --usesvm      use the host-mapped (_svm) calls
 //   -e, --emulate     the reference's FPGA emulator: not available for the GPU
 //   -h, --help
+//   --fftw-plan       estimate | measure | patient | exhaustive: FFTW planning
+//                     rigour of the CPU-only mode (the reference fixes it when
+//                     it is built, FFTW_PLAN in src/CMakeLists.txt:5-6;
+//                     default estimate as there)
 //
 // The CPU check and the CPU-only mode follow src/fft_conv3D.cpp:50-266 on
 // FFTW single precision, loaded at run time (libfftw3f.so.3 and its threads
@@ -36,7 +40,7 @@
 namespace {
 
 struct Config {
-  std::string path, wisdomfile = "a.out";
+  std::string path, wisdomfile = "a.out", fftw_plan = "estimate";
   unsigned num = 64, iter = 1, threads = 1, batch = 1;
   bool noverify = false, cpuonly = false, usesvm = false, emulate = false;
 };
@@ -52,7 +56,28 @@ double now_ms() {
 typedef float fftwf_cpx[2];
 typedef void* fftwf_plan_t;
 constexpr int kFftwForward = -1, kFftwBackward = 1;
-constexpr unsigned kFftwEstimate = 1u << 6;
+constexpr unsigned kFftwEstimate = 1u << 6, kFftwMeasure = 0u, kFftwPatient = 1u << 5, kFftwExhaustive = 1u << 3;
+constexpr unsigned kFftwWisdomOnly = 1u << 21;
+
+// planner flag of a --fftw-plan name and the label the reference prints
+// (src/fft_conv3D.cpp:72-83); false for an unknown name
+bool fftw_plan_flag(const std::string& name, unsigned* flag, const char** label) {
+  static const struct {
+    const char* name;
+    unsigned flag;
+    const char* label;
+  } modes[] = {{"estimate", kFftwEstimate, "Estimate"},
+               {"measure", kFftwMeasure, "Measure"},
+               {"patient", kFftwPatient, "Patient"},
+               {"exhaustive", kFftwExhaustive, "Exhaustive"}};
+  for (const auto& m : modes)
+    if (name == m.name) {
+      *flag = m.flag;
+      *label = m.label;
+      return true;
+    }
+  return false;
+}
 
 struct Fftw {
   void* lib = nullptr;
@@ -169,9 +194,12 @@ int cpu_only(Fftw& f, const Config& c) {
   fftwf_cpx* fs = (fftwf_cpx*)f.malloc_(sizeof(fftwf_cpx) * P);
   fftwf_cpx* ff = (fftwf_cpx*)f.malloc_(sizeof(fftwf_cpx) * P);
   unsigned flags = kFftwEstimate;
+  const char* label = "Estimate";
+  fftw_plan_flag(c.fftw_plan, &flags, &label);  // validated by parse_args
+  printf("-- Creating Plan\n-- FFTW Plan: %s\n", label);
   const bool wis = f.import_wisdom && f.import_wisdom(c.wisdomfile.c_str()) != 0;
   printf("-- %s wisdom from %s\n", wis ? "Importing" : "Cannot import", c.wisdomfile.c_str());
-  if (wis) flags |= 1u << 21;  // FFTW_WISDOM_ONLY
+  if (wis) flags = kFftwWisdomOnly | kFftwEstimate;  // plans come from the wisdom (fft_conv3D.cpp:85-92)
   const double t0 = now_ms();
   fftwf_plan_t pf = plan3(f, c.num, 1, ff, kFftwForward, flags);
   fftwf_plan_t ps = plan3(f, c.num, 1, fs, kFftwForward, flags);
@@ -267,7 +295,8 @@ void usage(const char* argv0) {
          "  -c, --cpu-only           CPU FFTW only\n"
          "  -s, --usesvm             host-mapped calls\n"
          "  -e, --emulate            FPGA emulation (not available)\n"
-         "  -h, --help               this text\n",
+         "  -h, --help               this text\n"
+         "      --fftw-plan <mode>   estimate | measure | patient | exhaustive (CPU-only mode; default estimate)\n",
          argv0);
 }
 
@@ -292,6 +321,7 @@ int parse_args(int argc, char** argv, Config& c) {
                                 {"usesvm", no_argument, nullptr, 's'},
                                 {"emulate", no_argument, nullptr, 'e'},
                                 {"help", no_argument, nullptr, 'h'},
+                                {"fftw-plan", required_argument, nullptr, 'P'},
                                 {nullptr, 0, nullptr, 0}};
   int o;
   while ((o = getopt_long(argc, argv, "p:w:n:i:t:yb:cseh", opts, nullptr)) != -1) {
@@ -308,10 +338,20 @@ int parse_args(int argc, char** argv, Config& c) {
       case 's': c.usesvm = true; break;
       case 'e': c.emulate = true; break;
       case 'h': usage(argv[0]); return 2;
+      case 'P': {
+        unsigned flag;
+        const char* label;
+        c.fftw_plan = optarg;
+        ok = fftw_plan_flag(c.fftw_plan, &flag, &label);
+        break;
+      }
       default: return 1;
     }
     if (!ok) {
-      fprintf(stderr, "Error parsing options: bad value '%s' for -%c\n", optarg, o);
+      if (o == 'P')
+        fprintf(stderr, "Error parsing options: bad value '%s' for --fftw-plan\n", optarg);
+      else
+        fprintf(stderr, "Error parsing options: bad value '%s' for -%c\n", optarg, o);
       return 1;
     }
   }
@@ -340,8 +380,11 @@ int main(int argc, char** argv) {
   if (pr == 2) return EXIT_SUCCESS;
   if (pr != 0) return EXIT_FAILURE;
   printf("\nCONFIGURATION: \n---------------\n");
-  printf("Code object  = %s\nPoints       = {%u, %u, %u}\nWisdom Path  = %s\nFFTW Plan    = Estimate\n",
-         c.path.c_str(), c.num, c.num, c.num, c.wisdomfile.c_str());
+  unsigned plan_flag;
+  const char* plan_label = "Estimate";
+  fftw_plan_flag(c.fftw_plan, &plan_flag, &plan_label);
+  printf("Code object  = %s\nPoints       = {%u, %u, %u}\nWisdom Path  = %s\nFFTW Plan    = %s\n",
+         c.path.c_str(), c.num, c.num, c.num, c.wisdomfile.c_str(), plan_label);
   printf("Threads      = %u\nIterations   = %u\nBatch        = %u\n----------------\n\n", c.threads, c.iter,
          c.batch);
 

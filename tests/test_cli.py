@@ -39,6 +39,15 @@ def test_bad_values_fail():
     assert _run("-n", "abc")[0] == 1
     assert _run("-i", "0")[0] == 1
     assert _run("extra")[0] == 1
+    assert _run("--fftw-plan", "fastest")[0] == 1
+
+
+def test_fftw_plan_mode_is_reported():
+    # FFTW_PLAN of the reference's build (src/CMakeLists.txt:5-6), chosen at run time here
+    for mode, label in (("estimate", "Estimate"), ("measure", "Measure"), ("patient", "Patient"),
+                        ("exhaustive", "Exhaustive")):
+        rc, out = _run("-c", "-n", "8", "--fftw-plan", mode)[:2]
+        assert "FFTW Plan    = %s" % label in out
 
 
 def test_non_svm_batch_is_rejected():
