@@ -2,7 +2,8 @@
  * convfpga_ext.h -- extensions of the drop-in API (not needed by callers of
  * convfpga.h).  Plain C ABI: opaque handles, pointers and sizes; device
  * pointers are HIP device addresses, `stream` is a hipStream_t passed as
- * void* (NULL = the library's own stream).  All calls require a successful
+ * void* (NULL = HIP's null stream, so the call is ordered with other work
+ * on it, e.g. torch's default stream).  All calls require a successful
  * fpga_initialize() and return 0 or a negative CF_ERR_* code; the message of
  * the last error on the calling thread is in cf_last_error().
  *
