@@ -473,6 +473,42 @@ def test_host_pipeline_256_equals_device_path(gpu, oracle, precision):
     assert np.array_equal(hout.view(np.uint32), want.view(np.uint32))
 
 
+def test_batch_graph_keeps_order_of_dependent_instances(gpu, oracle):
+    """Instances of a graph batch whose buffers overlap keep the order they were
+    given: a chain (the second reads the first's output) and two instances
+    writing one output (the last one wins), with unrelated instances between."""
+    import torch
+    api = gpu.api
+    n = 32
+    P = n ** 3
+    print("case batch graph dependent instances", flush=True)
+    s1, s2 = oracle.uniform(P, 71001), oracle.uniform(P, 71002)
+    f1, f2 = oracle.uniform(P, 71003), oracle.uniform(P, 71004)
+    sp1, sp2 = gpu.Spectrum(n, _t(f1)), gpu.Spectrum(n, _t(f2))
+    d1, mid, fin = _t(s1), torch.zeros(P, dtype=torch.complex64, device="cuda"), \
+        torch.zeros(P, dtype=torch.complex64, device="cuda")
+    # unrelated instances of other edges in between
+    others = [(_t(oracle.uniform(m ** 3, 71100 + m)), m) for m in (64, 16, 64)]
+    extra = [(x, torch.zeros_like(x), gpu.Spectrum(m, _t(oracle.uniform(m ** 3, 71200 + m)))) for x, m in others]
+    chain = api.Batch([(d1, mid, sp1)] + extra + [(mid, fin, sp2)], mode=api.CF_BATCH_GRAPH)
+    for rep in range(2):
+        mid.zero_()
+        fin.zero_()
+        chain.launch()
+        chain.check()
+        r1 = oracle.conv3d_f64(n, s1, f1, 1.0 / P)
+        r2 = oracle.conv3d_f64(n, np.ascontiguousarray(r1.astype(np.complex64)), f2, 1.0 / P)
+        assert oracle.rel_l2(_np(fin), r2) <= REL_L2_FP32, rep
+    chain.close()
+    d2 = _t(s2)
+    same = api.Batch([(d1, fin, sp1)] + extra + [(d2, fin, sp2)], mode=api.CF_BATCH_GRAPH)
+    fin.zero_()
+    same.launch()
+    same.check()
+    assert oracle.rel_l2(_np(fin), oracle.conv3d_f64(n, s2, f2, 1.0 / P)) <= REL_L2_FP32
+    same.close()
+
+
 def test_batch_shared_spectrum_in_place(gpu, oracle):
     n, k = 64, 4
     print("case batch shared spectrum in place", flush=True)
