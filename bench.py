@@ -193,6 +193,7 @@ def run_single(a, torch, C, api):
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
     line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, out)
+    line["slab_share"] = share_leg(C, n, spec, sig, out)
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
@@ -231,6 +232,25 @@ def power_leg(torch, C, spec, sig, out, seconds=2.0, idle_s=0.5):
            "energy_counter_mean_W": s.get("energy_mean_W"), "samples": s["samples"], "convs": count,
            "conv_per_s": count / s["seconds"], "conv_per_J": count / joules if joules else None}
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()}
+
+
+def share_leg(C, n, spec, sig, out, shares=8, reps=20):
+    """One GPU's share of the convolution decomposed over `shares` GPUs (slabs
+    for the outer passes, pencils for the passes along z): every pass on
+    1/shares of its tiles at the real shapes, exchanges not included.  The
+    exchange volume is stated, not timed."""
+    sums = None
+    for _ in range(reps):
+        ps = C.api.conv3d_share_timed(spec, sig, out, shares)
+        if sums is None:
+            sums = [[p["name"], 0.0, p["bytes"]] for p in ps]
+        for acc, p in zip(sums, ps):
+            acc[1] += p["ms"]
+    passes = [{"name": nm, "us": round(1e3 * ms / reps, 2), "GBps": b / (ms / reps * 1e-3) / 1e9}
+              for nm, ms, b in sums]
+    V = 8.0 * n ** 3
+    return {"shares": shares, "passes": passes, "us_sum": round(sum(p["us"] for p in passes), 2),
+            "all_to_all_bytes_out_per_gpu": V / shares * (shares - 1) / shares, "all_to_alls_per_conv": 2}
 
 
 def own_filter_leg(torch, C, n, sig, filt, out, reps=20):

@@ -39,7 +39,7 @@ DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpga
                   "fpgaf_conv3D_svm", "fpgaf_conv3D_svm_batch")
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
                "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device", "cf_conv3d_host",
-               "cf_conv3d_device_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
+               "cf_conv3d_device_timed", "cf_conv3d_share_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
                "cf_batch_destroy")
@@ -107,6 +107,7 @@ def lib():
             "cf_conv3d_device": (i, [vp, vp, vp, u, i, i, vp]),
             "cf_conv3d_host": (i, [vp, vp, vp, u, i, i, ctypes.POINTER(cf_host_times_t)]),
             "cf_conv3d_device_timed": (i, [vp, vp, vp, u, i, i, vp, ctypes.POINTER(cf_pass_times_t)]),
+            "cf_conv3d_share_timed": (i, [vp, vp, vp, u, i, vp, ctypes.POINTER(cf_pass_times_t)]),
             "cf_fft3d_device": (i, [u, vp, vp, u, i, vp]),
             "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
             "cf_profile_begin": (i, [u]),
@@ -280,6 +281,15 @@ def conv3d_device_timed(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, preci
     t = cf_pass_times_t()
     _check(lib().cf_conv3d_device_timed(spec.handle, _dev(sig), _dev(out), how_many, scale, precision,
                                         _stream(stream), ctypes.byref(t)), "cf_conv3d_device_timed")
+    return t.as_list()
+
+
+def conv3d_share_timed(spec, sig, out, shares, precision=CF_PREC_FP32, stream=None):
+    """Per-pass times of one GPU's share (1/shares of every pass's tiles) of a
+    slab-decomposed convolution of one cube (cf_conv3d_share_timed)."""
+    t = cf_pass_times_t()
+    _check(lib().cf_conv3d_share_timed(spec.handle, _dev(sig), _dev(out), shares, precision, _stream(stream),
+                                       ctypes.byref(t)), "cf_conv3d_share_timed")
     return t.as_list()
 
 

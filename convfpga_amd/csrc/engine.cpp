@@ -1339,6 +1339,59 @@ extern "C" int cf_conv3d_device_timed(cf_spectrum_t h, const void* d_sig, void* 
   return conv_device_impl(h, d_sig, d_out, how_many, scale_mode, precision, stream, times);
 }
 
+extern "C" int cf_conv3d_share_timed(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned shares,
+                                     int precision, void* stream, cf_pass_times_t* times) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!h || !d_sig || !d_out || !times) return fail(CF_ERR_ARG, "NULL pointer");
+  if (shares == 0 || shares > 16 || (shares & (shares - 1)))
+    return fail(CF_ERR_ARG, "shares must be 1, 2, 4, 8 or 16, not %u", shares);
+  CFCHK(check_conv_args(h, 1, CF_SCALE_INV_N3, precision));
+  CFCHK(require_ready());
+  SizeFns* f = nullptr;
+  CFCHK(size_res(h->n, &f));
+  const unsigned n = h->n;
+  hipStream_t s = (hipStream_t)stream;
+  void* work = nullptr;
+  if (is_split(h->plan)) CFCHK(acquire_work(cube_bytes(n), s, &work));
+  struct WorkRelease {
+    void* w;
+    hipStream_t s;
+    ~WorkRelease() {
+      if (w) release_work(s);
+    }
+  } wr{work, s};
+  std::vector<Pass> p;
+  conv_passes(*f, h->plan, n, 1, d_sig, d_out, work, h->d, 0, (float)(1.0 / (double)cube_elems(n)), p, precision);
+  // every work-group count stays a multiple of 8 x the widest sibling group (8)
+  for (const Pass& q : p)
+    if (shares > 1 && q.tiles % (shares * 64) != 0)
+      return fail(CF_ERR_ARG, "pass %s of N = %u has %u tiles: too few for %u shares", q.name, n, q.tiles, shares);
+  CFCHK(ensure_events());
+  HIPCHK(hipEventRecord(g.ev[0], s));
+  for (size_t i = 0; i < p.size(); ++i) {
+    const Pass& q = p[i];
+    const void* in = q.in;
+    void* out = q.out;
+    const void* H = q.H;
+    long long hs = q.hstride;
+    float sc = q.scale;
+    const void* twp = f->tw;
+    int ntotal = (int)(q.tiles / shares);
+    void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc, (void*)&ntotal};
+    HIPCHK(hipModuleLaunchKernel(q.f, (unsigned)ntotal, 1, 1, q.threads, 1, 1, 0, s, args, nullptr));
+    HIPCHK(hipEventRecord(g.ev[i + 1], s));
+  }
+  HIPCHK(hipEventSynchronize(g.ev[p.size()]));
+  memset(times, 0, sizeof *times);
+  times->npass = (unsigned)p.size();
+  for (size_t i = 0; i < p.size(); ++i) {
+    times->ms[i] = (float)ms_between(g.ev[i], g.ev[i + 1]);
+    snprintf(times->name[i], sizeof times->name[i], "%s", p[i].name);
+    times->bytes[i] = p[i].bytes / shares;
+  }
+  return CF_OK;
+}
+
 extern "C" int cf_profile_begin(unsigned max_calls) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (max_calls == 0 || max_calls > 100000) return fail(CF_ERR_ARG, "max_calls must be in 1..100000");

@@ -131,6 +131,18 @@ typedef struct {
 int cf_conv3d_device_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned how_many, int scale_mode,
                            int precision, void *stream, cf_pass_times_t *times);
 
+/* One GPU's share of a slab-decomposed convolution (design for several GPUs:
+ * each of `shares` GPUs holds N/shares z-planes for the x pass and the outer
+ * passes of the four-step plans, and after an all-to-all a 1/shares range of
+ * pencils for the passes along z).  Runs every pass of the spectrum's plan on
+ * one cube over its first 1/shares of tiles -- that share, at its real shape
+ * -- without the exchanges between the passes, and returns each pass's time
+ * and algorithmic bytes.  shares = 1, 2, 4, 8 or 16; with shares > 1 d_out and
+ * the workspace receive partial data, not a convolution; with shares = 1 it is
+ * cf_conv3d_device_timed of one cube with CF_SCALE_INV_N3.  Blocks. */
+int cf_conv3d_share_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned shares, int precision,
+                          void *stream, cf_pass_times_t *times);
+
 /* Event profiling of the next cf_conv3d_device calls without a host sync per
  * call: cf_profile_begin arms it for up to max_calls calls, cf_profile_end
  * waits for them and returns each pass's mean device time over the recorded

@@ -168,6 +168,35 @@ def test_device_batch_shared_and_own_spectra(gpu, oracle):
         assert oracle.rel_l2(np.ascontiguousarray(g2[s]), r2) <= REL_L2_FP32
 
 
+@pytest.mark.parametrize("plan", [1, 7])
+def test_share_timed(gpu, oracle, plan):
+    """cf_conv3d_share_timed: with one share it is the timed convolution of
+    one cube (same output bits); with 8 shares every pass runs on 1/8 of its
+    tiles and reports 1/8 of the bytes."""
+    import torch
+    api = gpu.api
+    n = 256
+    print("case share timed plan=%d" % plan, flush=True)
+    P = n ** 3
+    sig = torch.empty(P, dtype=torch.complex64, device="cuda")
+    filt = torch.empty_like(sig)
+    api.fill_uniform(sig, 2 * P, 3131)
+    api.fill_uniform(filt, 2 * P, 3132)
+    api.set_plan(n, plan)
+    spec = gpu.Spectrum(n, filt)
+    api.set_plan(n, api.CF_PLAN_AUTO)
+    a, b = torch.empty_like(sig), torch.empty_like(sig)
+    full = gpu.conv3d_device_timed(spec, sig, a)
+    one = api.conv3d_share_timed(spec, sig, b, 1)
+    assert torch.equal(a, b)
+    eight = api.conv3d_share_timed(spec, sig, b, 8)
+    assert [p["name"] for p in eight] == [p["name"] for p in full]
+    for p8, p1 in zip(eight, one):
+        assert p8["bytes"] * 8 == p1["bytes"] and 0 < p8["ms"] < p1["ms"]
+    with pytest.raises(api.ConvError):
+        api.conv3d_share_timed(spec, sig, b, 3)
+
+
 def test_256_conv_matches_oracle(gpu, oracle):
     n = 256
     print("case 256^3 conv vs FP64 oracle", flush=True)
