@@ -83,11 +83,18 @@ def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
         O.conv3d_f32_spec(n, sig, H, out)  # input re-copied every rep
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
+    # one thread as well (the reference CLI's default, src/helper.cpp:68), on fewer reps
+    t1 = []
+    for _ in range(max(1, min(2, reps))):
+        t0 = time.perf_counter()
+        O.conv3d_f32_spec(n, sig, H, out, nthreads=1)
+        t1.append(time.perf_counter() - t0)
     res = {
         "value": 1.0 / med, "unit": "conv/s", "cores": O.max_threads(), "kind": "port",
         "sample": "%d reps of one %d^3 FP32 convolution (FFT, product, inverse FFT; filter spectrum "
                   "precomputed), median; oracle's own radix-2 FFT with OpenMP (FFTW3 %s)"
                   % (reps, n, "present" if O.fftw_available() else "not installed"),
+        "value_1_thread": 1.0 / statistics.median(t1), "cpu_model": cpu_model(),
     }
     check = {}
     if gpu_out_host is not None:
@@ -97,6 +104,16 @@ def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
         ref64 = O.conv3d_f64(n, sig, filt, 1.0 / P)
         check["rel_l2_vs_fp64"] = O.rel_l2(first, ref64)
     return res, check
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_bandwidth(torch, nelem, reps=5):
