@@ -28,10 +28,12 @@ CF_MAX_PASSES = 8
 CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
 CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
 CF_PLAN_SPLIT3_G24M2, CF_PLAN_SPLIT3_G24M1, CF_PLAN_SPLIT3_G28M4, CF_PLAN_SPLIT3_G28M2, CF_PLAN_SPLIT3_G28M1 = 7, 8, 9, 10, 11
-CF_NUM_PLANS = 12
+CF_PLAN_AXIS5Z = 12
+CF_NUM_PLANS = 13
 CF_BATCH_PERSISTENT, CF_BATCH_GRAPH = 0, 1
 PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44",
-              7: "split3_g24m2", 8: "split3_g24m1", 9: "split3_g28m4", 10: "split3_g28m2", 11: "split3_g28m1"}
+              7: "split3_g24m2", 8: "split3_g24m1", 9: "split3_g28m4", 10: "split3_g28m2", 11: "split3_g28m1",
+              12: "axis5z"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
 # names of every symbol include/convfpga/*.h declares

@@ -65,6 +65,7 @@ struct SizeFns {
   hipFunction_t row_fwd = nullptr, row_inv = nullptr, coly_fwd = nullptr, coly_inv = nullptr;
   hipFunction_t colz_fwd = nullptr, colz_inv = nullptr, colz_conv = nullptr;
   hipFunction_t plane_fwd = nullptr, plane_inv = nullptr;                 // N in {32, 64, 128}
+  hipFunction_t coly_fwd_zb = nullptr, coly_inv_zb = nullptr, colz_fwd_zb = nullptr, colz_conv_zb = nullptr;  // N >= 64
   hipFunction_t xy1_fwd[cfgeo::kSplitVariants] = {}, xy1_inv[cfgeo::kSplitVariants] = {};  // N = 256
   hipFunction_t mid[cfgeo::kSplitVariants] = {}, hlay[cfgeo::kSplitVariants] = {};
   hipFunction_t mx_xy1_fwd = nullptr, mx_xy1_inv = nullptr, mx_mid = nullptr;  // N = 256, split-FP16 MFMA
@@ -75,6 +76,8 @@ struct SizeFns {
 };
 
 bool is_split(int plan) { return plan >= CF_PLAN_SPLIT3_G22 && plan < CF_PLAN_SPLIT3_G22 + cfgeo::kSplitVariants; }
+// plans whose passes go through the workspace (one cube per convolution in flight)
+bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
 int split_variant(int plan) { return plan - CF_PLAN_SPLIT3_G22; }
 
 bool plan_ok(unsigned n, int plan) {
@@ -82,6 +85,7 @@ bool plan_ok(unsigned n, int plan) {
   if (plan == CF_PLAN_AXIS5) return true;
   if (plan == CF_PLAN_PLANE3) return cfgeo::plane_supported((int)n);
   if (is_split(plan)) return n == 256;
+  if (plan == CF_PLAN_AXIS5Z) return n >= 64 && cfgeo::supported((int)n);
   return false;
 }
 
@@ -254,6 +258,19 @@ int size_res(unsigned n, SizeFns** out) {
     snprintf(name, sizeof name, t.fmt, n);
     CFCHK(get_fn(t.f, name));
   }
+  if (plan_ok(n, CF_PLAN_AXIS5Z)) {
+    struct {
+      hipFunction_t* f;
+      const char* fmt;
+    } zb[] = {{&s.coly_fwd_zb, "cf_coly_fwd_zb_%u"},
+              {&s.coly_inv_zb, "cf_coly_inv_zb_%u"},
+              {&s.colz_fwd_zb, "cf_colz_fwd_zb_%u"},
+              {&s.colz_conv_zb, "cf_colz_conv_zb_%u"}};
+    for (auto& t : zb) {
+      snprintf(name, sizeof name, t.fmt, n);
+      CFCHK(get_fn(t.f, name));
+    }
+  }
   if (cfgeo::plane_supported((int)n)) {
     snprintf(name, sizeof name, "cf_plane_fwd_%u", n);
     CFCHK(get_fn(&s.plane_fwd, name));
@@ -394,6 +411,15 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
     v.push_back({f.xy1_inv[vi], aT, at, work, out, nullptr, 0, scale, "xy1_inv", io});
     return split;
   }
+  if (plan == CF_PLAN_AXIS5Z) {  // natural -> z-blocked in the workspace -> natural
+    v.push_back({f.row_fwd, rT, rt, sig, out, nullptr, 0, 1.0f, "x_fwd", io});
+    v.push_back({f.coly_fwd_zb, cT, ct, out, work, nullptr, 0, 1.0f, "y_fwd", io});
+    v.push_back({f.colz_conv_zb, cT, ct, work, work, H, hstride, 1.0f, "z_fwd_mul_z_inv", io + hbytes});
+    const size_t split = v.size();
+    v.push_back({f.coly_inv_zb, cT, ct, work, out, nullptr, 0, 1.0f, "y_inv", io});
+    v.push_back({f.row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv", io});
+    return split;
+  }
   v.push_back({f.row_fwd, rT, rt, sig, out, nullptr, 0, 1.0f, "x_fwd", io});
   v.push_back({f.coly_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "y_fwd", io});
   v.push_back({f.colz_conv, cT, ct, out, out, H, hstride, 1.0f, "z_fwd_mul_z_inv", io + hbytes});
@@ -407,6 +433,15 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
 // re-layout into the middle pass's order for the four-step plans (via work).
 void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, const void* src, void* dst, void* work,
                      std::vector<Pass>& v) {
+  if (plan == CF_PLAN_AXIS5Z) {  // x into the workspace, y into dst (z-blocked), z in place
+    const double io = 2.0 * cube_bytes(n) * count;
+    const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
+    const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+    v.push_back({f.row_fwd, rT, rt, src, work, nullptr, 0, 1.0f, "x_fwd", io});
+    v.push_back({f.coly_fwd_zb, cT, ct, work, dst, nullptr, 0, 1.0f, "y_fwd", io});
+    v.push_back({f.colz_fwd_zb, cT, ct, dst, dst, nullptr, 0, 1.0f, "z_fwd", io});
+    return;
+  }
   if (!is_split(plan)) {
     fwd_passes(f, n, count, src, dst, v);
     return;
@@ -885,7 +920,7 @@ int pipe_feed(const PipeJob& j, PipeSync& ps, std::vector<hipEvent_t>& tev) {
     std::vector<Pass> p;
     const void* H = (const char*)j.H + (size_t)k0 * j.hstride * 8;
     void* work = nullptr;
-    if (is_split(j.plan)) {
+    if (needs_work(j.plan)) {
       const int rc = acquire_work(slot_bytes, j.s, &work);
       if (rc != CF_OK) return stop(rc);
     }
@@ -1029,7 +1064,7 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
     // overlapped pipeline; times as the reference's batch call reports them
     // (convfpga_svm_batch.c:664-665): summed pass spans and the wall span
     void* fwork = nullptr;
-    if (is_split(plan)) CFCHK(acquire_work(V, s, &fwork));
+    if (needs_work(plan)) CFCHK(acquire_work(V, s, &fwork));
     CFCHK(staged_h2d(g.d_filt, filter, V, s));
     spectrum_passes(*f, plan, n, 1, g.d_filt, g.d_filt, fwork, filt_p);
     const size_t nf = filt_p.size();
@@ -1047,7 +1082,7 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   }
   CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
   void* work = nullptr;
-  if (is_split(plan)) CFCHK(acquire_work(SV, s, &work));
+  if (needs_work(plan)) CFCHK(acquire_work(SV, s, &work));
 
   if (mode == CFE_MODE_COPY) {
     // Blocking phases as in the reference (convfpga.c:75-375); each host
@@ -1158,7 +1193,7 @@ extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filter
   int rc = e == hipSuccess ? choose_plan(n, f, &h->plan)
                            : fail(CF_ERR_RUNTIME, "hipMemcpyAsync: %s", hipGetErrorString(e));
   void* work = nullptr;
-  if (rc == CF_OK && is_split(h->plan)) rc = acquire_work(bytes, s, &work);
+  if (rc == CF_OK && needs_work(h->plan)) rc = acquire_work(bytes, s, &work);
   if (rc == CF_OK) {
     std::vector<Pass> p;
     spectrum_passes(*f, h->plan, n, count, src, h->d, work, p);
@@ -1264,7 +1299,7 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
   const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
   hipStream_t s = (hipStream_t)stream;
   void* work = nullptr;
-  if (is_split(h->plan)) CFCHK(acquire_work(cube_bytes(n) * how_many, s, &work));
+  if (needs_work(h->plan)) CFCHK(acquire_work(cube_bytes(n) * how_many, s, &work));
   std::vector<Pass> p;
   conv_passes(*f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p, precision);
   struct WorkRelease {  // the workspace is marked busy on s whatever path returns
@@ -1352,7 +1387,7 @@ extern "C" int cf_conv3d_share_timed(cf_spectrum_t h, const void* d_sig, void* d
   const unsigned n = h->n;
   hipStream_t s = (hipStream_t)stream;
   void* work = nullptr;
-  if (is_split(h->plan)) CFCHK(acquire_work(cube_bytes(n), s, &work));
+  if (needs_work(h->plan)) CFCHK(acquire_work(cube_bytes(n), s, &work));
   struct WorkRelease {
     void* w;
     hipStream_t s;
@@ -1546,10 +1581,10 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
     if (mode == CF_BATCH_PERSISTENT) {
       if (x.h->n < 16 || x.h->n > 256)
         return fail(CF_ERR_SIZE, "instance %u: the persistent batch runs edges 16..256, not %u", i, x.h->n);
-      if (is_split(x.h->plan))
-        return fail(CF_ERR_ARG, "instance %u: spectrum laid out for a four-step plan; create it under "
-                    "CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
-    } else if (is_split(x.h->plan)) {
+      if (needs_work(x.h->plan))
+        return fail(CF_ERR_ARG, "instance %u: spectrum in the private layout of a four-step or z-blocked plan; "
+                    "create it under CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
+    } else if (needs_work(x.h->plan)) {
       work_need = std::max(work_need, cube_bytes(x.h->n));
     }
   }
@@ -1633,7 +1668,7 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
     for (unsigned i = 0; i < count && rc == CF_OK;) {
       const unsigned n = inst[i].h->n;
       const size_t V = cube_bytes(n);
-      const bool splitp = is_split(inst[i].h->plan);
+      const bool splitp = needs_work(inst[i].h->plan);
       unsigned run = 1;
       const bool shared = i + 1 < count && inst[i + 1].h == inst[i].h;
       while (i + run < count && !splitp && inst[i + run].h->n == n && inst[i + run].h->plan == inst[i].h->plan &&

@@ -69,13 +69,34 @@ struct RowPass {
   }
 };
 
+// Cube layouts of the column passes.  Natural: z*N*N + y*N + x.  Z-blocked
+// (the intermediate of CF_PLAN_AXIS5Z): ((x/16)*N + y)*16*N + z*16 + x%16, so
+// that the 16 adjacent x-columns of one y hold their z-lines in one
+// contiguous block of 16*N points.
+template <int N, int AXIS, bool ZB>
+struct ColAddr {
+  static constexpr int C = N < 16 ? N : 16;
+  static_assert(!ZB || C == 16, "the z-blocked layout groups 16 x-columns");
+  // element stride along the axis
+  static constexpr int S = ZB ? (AXIS == 1 ? 16 * N : 16) : (AXIS == 1 ? N : N * N);
+  // offset of point 0 of the line of column x = xg*C + c at `other` (z for the y pass, y for the z pass)
+  __device__ __forceinline__ static int base(int xg, int c, int other) {
+    if constexpr (ZB)
+      return AXIS == 1 ? xg * N * 16 * N + other * 16 + c : (xg * N + other) * 16 * N + c;
+    else
+      return xg * C + c + (AXIS == 1 ? other * N * N : other * N);
+  }
+};
+
 // y (AXIS 1) or z (AXIS 2) pass.  MODE 0 forward, 1 inverse, 2 forward * H
-// then inverse (z only).
-template <int N, int AXIS, int MODE>
+// then inverse (z only).  IN_ZB / OUT_ZB: the input / output (and, for MODE 2,
+// the spectrum, which shares the output layout) are z-blocked.
+template <int N, int AXIS, int MODE, bool IN_ZB = false, bool OUT_ZB = false>
 struct ColPass {
   using K = ColCfg<N>;
+  using AI = ColAddr<N, AXIS, IN_ZB>;
+  using AO = ColAddr<N, AXIS, OUT_ZB>;
   static constexpr int P = K::P, T = K::T, C = K::C, G = K::G;
-  static constexpr int S = AXIS == 1 ? N : N * N;  // element stride along the axis
   struct Regs {
     float2 v[P];
   };
@@ -87,33 +108,34 @@ struct ColPass {
   long long hstride;
   int c, t, g;
 
+  template <class A>
   __device__ __forceinline__ int offset(int tt) const {
     const int xg = tt % (N / C);
     const int other = (tt / (N / C)) * G + g;  // z for the y pass, y for the z pass
-    return xg * C + c + (AXIS == 1 ? other * N * N : other * N) + t * S;
+    return A::base(xg, c, other) + t * A::S;
   }
   __device__ __forceinline__ void load(int tile, Regs& r) const {
     const int cube = tile / K::TILES, tt = tile % K::TILES;
-    const float2* src = in + (size_t)cube * N * N * N + offset(tt);
+    const float2* src = in + (size_t)cube * N * N * N + offset<AI>(tt);
 #pragma unroll
-    for (int m = 0; m < P; ++m) r.v[m] = src[T * S * m];
+    for (int m = 0; m < P; ++m) r.v[m] = src[T * AI::S * m];
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
     const int cube = tile / K::TILES, tt = tile % K::TILES;
-    const int off = offset(tt);
+    const int off = offset<AO>(tt);
     const ColLds<C> view{lds + g * N * (C + 1) + c};
     if constexpr (MODE == 2) {
       fft_line<N, false>(r.v, t, view, tw_s);
       const float2* hp = H + (size_t)cube * hstride + off;
 #pragma unroll
-      for (int m = 0; m < P; ++m) r.v[m] = cmul(r.v[m], hp[T * S * m]);
+      for (int m = 0; m < P; ++m) r.v[m] = cmul(r.v[m], hp[T * AO::S * m]);
       fft_line<N, true>(r.v, t, view, tw_s);
     } else {
       fft_line<N, MODE == 1>(r.v, t, view, tw_s);
     }
     float2* dst = out + (size_t)cube * N * N * N + off;
 #pragma unroll
-    for (int m = 0; m < P; ++m) dst[T * S * m] = r.v[m];
+    for (int m = 0; m < P; ++m) dst[T * AO::S * m] = r.v[m];
   }
 };
 
@@ -127,7 +149,7 @@ __device__ __forceinline__ void row_kernel(const float2* in, float2* out, const 
   run_tiles(k, ntotal);
 }
 
-template <int N, int AXIS, int MODE>
+template <int N, int AXIS, int MODE, bool IN_ZB = false, bool OUT_ZB = false>
 __device__ __forceinline__ void col_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
   using K = ColCfg<N>;
@@ -135,7 +157,7 @@ __device__ __forceinline__ void col_kernel(const float2* in, float2* out, const 
   __shared__ float2 tw_s[N];
   stage_twiddles<N, K::THREADS>(tw_s, tw);
   const int c = threadIdx.x % K::C, t = (threadIdx.x / K::C) % K::T, g = threadIdx.x / (K::C * K::T);
-  const ColPass<N, AXIS, MODE> k{in, out, H, tw_s, lds, hstride, c, t, g};
+  const ColPass<N, AXIS, MODE, IN_ZB, OUT_ZB> k{in, out, H, tw_s, lds, hstride, c, t, g};
   run_tiles(k, ntotal);
 }
 

@@ -53,6 +53,23 @@ struct GeoCheck {
     cfk::col_kernel<N, 2, 2>(in, out, H, tw, hstride, ntotal);                                      \
   }
 
+// five-pass schedule with a z-blocked intermediate (CF_PLAN_AXIS5Z): the y pass
+// writes the layout ((x/16)*N + y)*16*N + z*16 + x%16, the z pass runs on
+// contiguous blocks of 16 z-lines, the inverse y pass returns to natural order
+#define CF_INSTANTIATE_ZB(N)                                                                          \
+  extern "C" __global__ __launch_bounds__(cfk::ColCfg<N>::THREADS) void cf_coly_fwd_zb_##N(CF_ARGS) {  \
+    cfk::col_kernel<N, 1, 0, false, true>(in, out, H, tw, hstride, ntotal);                             \
+  }                                                                                                     \
+  extern "C" __global__ __launch_bounds__(cfk::ColCfg<N>::THREADS) void cf_coly_inv_zb_##N(CF_ARGS) {  \
+    cfk::col_kernel<N, 1, 1, true, false>(in, out, H, tw, hstride, ntotal);                             \
+  }                                                                                                     \
+  extern "C" __global__ __launch_bounds__(cfk::ColCfg<N>::THREADS) void cf_colz_fwd_zb_##N(CF_ARGS) {  \
+    cfk::col_kernel<N, 2, 0, true, true>(in, out, H, tw, hstride, ntotal);                              \
+  }                                                                                                     \
+  extern "C" __global__ __launch_bounds__(cfk::ColCfg<N>::THREADS) void cf_colz_conv_zb_##N(CF_ARGS) { \
+    cfk::col_kernel<N, 2, 2, true, true>(in, out, H, tw, hstride, ntotal);                              \
+  }
+
 CF_INSTANTIATE(8)
 CF_INSTANTIATE(16)
 CF_INSTANTIATE(32)
@@ -60,6 +77,10 @@ CF_INSTANTIATE(64)
 CF_INSTANTIATE(128)
 CF_INSTANTIATE(256)
 CF_INSTANTIATE(512)
+CF_INSTANTIATE_ZB(64)
+CF_INSTANTIATE_ZB(128)
+CF_INSTANTIATE_ZB(256)
+CF_INSTANTIATE_ZB(512)
 
 // ---- 3-pass schedules (three_pass.h) ------------------------------------------
 #define CF_PLANE(N)                                                                                   \

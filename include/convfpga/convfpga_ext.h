@@ -54,7 +54,9 @@ extern "C" {
 #define CF_PLAN_SPLIT3_G28M4 9
 #define CF_PLAN_SPLIT3_G28M2 10
 #define CF_PLAN_SPLIT3_G28M1 11
-#define CF_NUM_PLANS 12
+#define CF_PLAN_AXIS5Z 12 /* the five passes with a z-blocked intermediate in the workspace: the
+                             z pass reads and writes contiguous blocks; N in {64,...,512}   */
+#define CF_NUM_PLANS 13
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
@@ -83,7 +85,8 @@ int cf_spectrum_create(unsigned N, unsigned count, const void *filters, int on_d
                        cf_spectrum_t *out);
 void cf_spectrum_destroy(cf_spectrum_t h);
 /* device address of the spectra: count * N^3 complex floats, natural order for
- * CF_PLAN_AXIS5 / CF_PLAN_PLANE3, an internal blocked order for CF_PLAN_SPLIT3_* */
+ * CF_PLAN_AXIS5 / CF_PLAN_PLANE3, an internal blocked order for CF_PLAN_SPLIT3_*
+ * and CF_PLAN_AXIS5Z */
 const void *cf_spectrum_data(cf_spectrum_t h);
 unsigned cf_spectrum_edge(cf_spectrum_t h);
 unsigned cf_spectrum_count(cf_spectrum_t h);

@@ -267,7 +267,7 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
-    [(256, p) for p in range(3, 12)] + [(512, 1)]
+    [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
@@ -309,7 +309,7 @@ def test_every_plan_matches_oracle(gpu, oracle, n, plan):
         api.set_plan(n, api.CF_PLAN_AUTO)
 
 
-@pytest.mark.parametrize("plan", [1, 3, 6])
+@pytest.mark.parametrize("plan", [1, 3, 6, 12])
 def test_dropin_256_on_each_plan(gpu, oracle, plan):
     n = 256
     print("case drop-in N=256 plan=%d" % plan, flush=True)
@@ -380,16 +380,15 @@ def _check_instances(oracle, inst):
         assert oracle.rel_l2(got, ref) <= REL_L2_FP32, n
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batch_mixed_sizes_replayed(gpu, oracle, mode):
+@pytest.mark.parametrize("mode,plan256", [(0, 1), (1, 4), (1, 12)])
+def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
     """Mixed edges with their own filters in one graph launch (persistent kernel
     or recorded per-instance passes); replayed with eager work in between, every
-    replay checked."""
+    replay checked.  The 256^3 instance runs the given plan."""
     import torch
     api = gpu.api
     sizes = [32, 64, 16, 128, 32, 256, 64, 32]
-    print("case batch mode=%d sizes=%s" % (mode, sizes), flush=True)
-    plan256 = api.CF_PLAN_AXIS5 if mode == api.CF_BATCH_PERSISTENT else api.CF_PLAN_SPLIT3_G24
+    print("case batch mode=%d plan256=%d sizes=%s" % (mode, plan256, sizes), flush=True)
     inst = _mixed_instances(gpu, oracle, sizes, plan256)
     b = gpu.api.Batch([(ds, do, spec) for ds, do, spec, _, _ in inst], mode=mode)
     for rep in range(3):
@@ -576,15 +575,18 @@ def test_batch_stalled_dependency_is_reported(gpu, oracle, monkeypatch):
     ok.close()
 
 
-def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle):
+@pytest.mark.parametrize("plan,what", [(4, "four-step"), (12, "z-blocked")])
+def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle, plan, what):
+    """The persistent batch reads spectra in natural order: spectra in the
+    private layouts of the four-step and z-blocked plans are refused."""
     api = gpu.api
-    api.set_plan(256, api.CF_PLAN_SPLIT3_G24)
+    api.set_plan(256, plan)
     try:
         spec = gpu.Spectrum(256, _t(oracle.uniform(256 ** 3, 72001)))
     finally:
         api.set_plan(256, api.CF_PLAN_AUTO)
     d = _t(oracle.uniform(256 ** 3, 72002))
-    with pytest.raises(gpu.ConvError, match="four-step"):
+    with pytest.raises(gpu.ConvError, match=what):
         api.Batch([(d, d, spec)], mode=api.CF_BATCH_PERSISTENT)
 
 
