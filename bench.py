@@ -456,7 +456,8 @@ def run_split(a, torch, C, api):
     """4 x 256^3 + 1 x 512^3 convolutions with their own filters, for U[-1,1)
     and for log-uniform inputs (both signal and filter): the split-FP16 MFMA
     form beside the FP32 form.  The MFMA form exists for the 256^3 four-step
-    kernels; the 512^3 instance runs the FP32 5-pass schedule in both columns.
+    kernels; the 512^3 instance runs the FP32 five-pass schedule of the
+    autotuned plan (the z-blocked one on MI355X) in both columns.
     A step is the whole batch (one call for the four 256^3 cubes, one for the
     512^3 cube), inputs resident in HBM, filter spectra precomputed."""
     n1, b1, n2 = 256, 4, 512
@@ -477,7 +478,7 @@ def run_split(a, torch, C, api):
                 api.fill_loguniform(x, x.numel(), seed + k)
         torch.cuda.synchronize()
         spec1 = C.Spectrum(n1, flt1, count=b1)
-        spec2 = C.Spectrum(n2, flt2)
+        spec2 = C.Spectrum(n2, flt2)  # autotuned plan for 512^3
         out1, out2 = torch.empty_like(sig1), torch.empty_like(sig2)
         entry = {}
         host = {"sig256": sig1.cpu().numpy(), "flt256": flt1.cpu().numpy(),
@@ -515,13 +516,14 @@ def run_split(a, torch, C, api):
         del spec1, spec2, sig1, flt1, sig2, flt2, out1, out2
         torch.cuda.empty_cache()
     api.set_plan(n1, api.CF_PLAN_AUTO)
+    plan512 = api.PLAN_NAMES[api.get_plan(n2)]
     V1, V2 = 8.0 * P1, 8.0 * P2
     alg = b1 * 7 * V1 + 11 * V2  # executed schedules: four-step 256^3 (7V), 5-pass 512^3 (11V)
     u = dists["uniform"]["split_fp16_mfma"]
     line = base_line(a, "split-MFMA vs FP32 FFT convolution, 4x256^3 + 1x512^3 (own filters)", u["conv_per_s"],
                      "conv/s", u["ms_per_batch"],
                      {"workload": "c5_split_4x256^3+1x512^3", "instances": b1 + 1, "filters": "own per instance",
-                      "scale": "1/N^3", "plans": {"256": "split3_g24m2", "512": "axis5"},
+                      "scale": "1/N^3", "plans": {"256": "split3_g24m2", "512": plan512},
                       "split_fp16_sizes": [256]})
     line["dtype"] = "split-fp16 mfma (fp32 accumulate) for 256^3; fp32 for 512^3"
     line["data"] = "synthetic: U[-1,1) and log-uniform magnitudes 1e-3..1e3 with uniform phase, generated on the GPU"
