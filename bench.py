@@ -185,6 +185,13 @@ def run_single(a, torch, C, api):
     passes, ncalls = api.profile_end()
     assert ncalls == a.steps
     wall = t1 - t0
+    # the same loop without the per-pass events (each event packet is a short
+    # gap on the GPU: a few percent at 256^3, more for launch-bound small cubes)
+    t2 = time.perf_counter()
+    for _ in range(a.steps):
+        C.conv3d_device(spec, sig, out)
+    torch.cuda.synchronize()
+    wall_plain = time.perf_counter() - t2
     alg_bytes = sum(p["bytes"] for p in passes)  # per step
     for p in passes:
         p["GBps"] = p["bytes"] / (p["ms"] * 1e-3) / 1e9
@@ -206,6 +213,7 @@ def run_single(a, torch, C, api):
     # the same rate priced with the 5-pass model: 11 volume transfers per convolution
     line["conv_roofline_frac_5pass_model"] = 11 * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS
     line["device_ms_per_step"] = sum(p["ms"] for p in passes)
+    line["conv_per_s_without_pass_events"] = a.steps / wall_plain
     line["copy_kernel_GBps"] = copy_gbs
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
