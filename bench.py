@@ -16,7 +16,7 @@ convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
 against the FP64 oracle.
 
-Other workloads (reported lines, not the metric): c2_128 / c1_64 (one cube),
+Other workloads (reported lines, not the metric): c2_128 / c1_64 / c5_512 (one cube),
 c3_64x64 (64 cubes of 64^3, one shared spectrum), c4_mixed (16x32^3 + 8x64^3 +
 4x128^3 + 1x256^3, own spectra, shuffled), each comparing the persistent
 one-launch kernel, the recorded graph and, where it applies, batched passes;
@@ -38,7 +38,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 
-SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64}
+SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64, "c5_512": 512}
 SPLIT = {"c5_split": [256] * 4 + [512]}
 BATCHES = {
     "c3_64x64": [64] * 64,
@@ -149,7 +149,10 @@ KERNEL_OF_PASS = {  # kernel of each pass name (N = edge; G, GM = tile-group suf
 
 def kernel_of_pass(name, n, plan_name):
     gm = "_" + plan_name.split("_")[1] if plan_name.startswith("split3_") else ""
-    return KERNEL_OF_PASS.get(name, "").format(N=n, G=gm.split("m")[0], GM=gm)
+    k = KERNEL_OF_PASS.get(name, "").format(N=n, G=gm.split("m")[0], GM=gm)
+    if plan_name == "axis5z" and name in ("y_fwd", "y_inv", "z_fwd_mul_z_inv"):
+        k = k.replace("_%d" % n, "_zb_%d" % n)  # the z-blocked column kernels
+    return k
 
 
 def run_single(a, torch, C, api):
