@@ -268,6 +268,10 @@ def share_leg(C, n, spec, sig, out, shares=8, reps=20):
     1/shares of its tiles at the real shapes, exchanges not included.  The
     exchange volume is stated, not timed."""
     sums = None
+    try:
+        C.api.conv3d_share_timed(spec, sig, out, shares)
+    except C.api.ConvError as e:  # plans with too few tiles per pass to split (small cubes)
+        return {"shares": shares, "unavailable": str(e)}
     for _ in range(reps):
         ps = C.api.conv3d_share_timed(spec, sig, out, shares)
         if sums is None:
