@@ -392,13 +392,14 @@ def run_batch(a, torch, C, api):
         return res
 
     # graph mode: spectra of each edge's autotuned plan; persistent mode:
-    # spectra in natural order (its layout), i.e. the axis plan for 256
+    # spectra in natural order (its layout), i.e. the axis plan for 128 and 256
     inst_g = instances(make_specs())
-    if 256 in by_n:
-        api.set_plan(256, api.CF_PLAN_AXIS5)
+    natural = [n for n in by_n if n >= 128]  # edges whose autotuned plan may lay spectra out privately
+    for n in natural:
+        api.set_plan(n, api.CF_PLAN_AXIS5)
     inst = instances(make_specs())
-    if 256 in by_n:
-        api.set_plan(256, api.CF_PLAN_AUTO)
+    for n in natural:
+        api.set_plan(n, api.CF_PLAN_AUTO)
     # graph mode records contiguous runs of one spectrum as batched passes: order by edge
     inst_sorted = sorted(inst_g, key=lambda t: (t[2].N, t[0].data_ptr()))
     results = {}

@@ -84,7 +84,7 @@ bool plan_ok(unsigned n, int plan) {
   if (!cfgeo::supported((int)n)) return false;
   if (plan == CF_PLAN_AXIS5) return true;
   if (plan == CF_PLAN_PLANE3) return cfgeo::plane_supported((int)n);
-  if (is_split(plan)) return n == 256;
+  if (is_split(plan)) return cfgeo::split_supported((int)n, split_variant(plan));
   if (plan == CF_PLAN_AXIS5Z) return n >= 64 && cfgeo::supported((int)n);
   return false;
 }
@@ -277,23 +277,24 @@ int size_res(unsigned n, SizeFns** out) {
     snprintf(name, sizeof name, "cf_plane_inv_%u", n);
     CFCHK(get_fn(&s.plane_inv, name));
   }
+  for (int vi = 0; vi < cfgeo::kSplitVariants; ++vi) {
+    if (!cfgeo::split_supported((int)n, vi)) continue;
+    const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
+    char sfx[16];
+    if (gm == gb)
+      snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
+    else
+      snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
+    snprintf(name, sizeof name, "cf_xy1_fwd_%u_g%d%d", n, ga, gb);
+    CFCHK(get_fn(&s.xy1_fwd[vi], name));
+    snprintf(name, sizeof name, "cf_xy1_inv_%u_g%d%d", n, ga, gb);
+    CFCHK(get_fn(&s.xy1_inv[vi], name));
+    snprintf(name, sizeof name, "cf_mid_%u_%s", n, sfx);
+    CFCHK(get_fn(&s.mid[vi], name));
+    snprintf(name, sizeof name, "cf_hlayout_%u_%s", n, sfx);
+    CFCHK(get_fn(&s.hlay[vi], name));
+  }
   if (n == 256) {
-    for (int vi = 0; vi < cfgeo::kSplitVariants; ++vi) {
-      const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
-      char sfx[16];
-      if (gm == gb)
-        snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
-      else
-        snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
-      snprintf(name, sizeof name, "cf_xy1_fwd_256_g%d%d", ga, gb);
-      CFCHK(get_fn(&s.xy1_fwd[vi], name));
-      snprintf(name, sizeof name, "cf_xy1_inv_256_g%d%d", ga, gb);
-      CFCHK(get_fn(&s.xy1_inv[vi], name));
-      snprintf(name, sizeof name, "cf_mid_256_%s", sfx);
-      CFCHK(get_fn(&s.mid[vi], name));
-      snprintf(name, sizeof name, "cf_hlayout_256_%s", sfx);
-      CFCHK(get_fn(&s.hlay[vi], name));
-    }
     CFCHK(get_fn(&s.mx_xy1_fwd, "cf_xy1_fwd_256_s24"));
     CFCHK(get_fn(&s.mx_xy1_inv, "cf_xy1_inv_256_s24"));
     CFCHK(get_fn(&s.mx_mid, "cf_mid_256_s24m2"));
@@ -403,8 +404,8 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
   }
   if (is_split(plan)) {
     const int vi = split_variant(plan);
-    const unsigned at = cfgeo::split_a_threads(vi), aT = cfgeo::split_a_tiles(vi);
-    const unsigned bt = cfgeo::split_b_threads(vi), bT = cfgeo::split_b_tiles(vi);
+    const unsigned at = cfgeo::split_a_threads((int)n, vi), aT = cfgeo::split_a_tiles((int)n, vi);
+    const unsigned bt = cfgeo::split_b_threads((int)n, vi), bT = cfgeo::split_b_tiles((int)n, vi);
     v.push_back({f.xy1_fwd[vi], aT, at, sig, work, nullptr, 0, 1.0f, "xy1_fwd", io});
     v.push_back({f.mid[vi], bT, bt, work, work, H, hstride, 1.0f, "y2z_fwd_mul_inv", io + hbytes});
     const size_t split = v.size();

@@ -32,18 +32,20 @@ constexpr bool plane_supported(int n) { return n == 32 || n == 64 || n == 128; }
 constexpr int plane_threads(int n) { return n * threads_per_line(n); }
 constexpr int plane_tiles(int n) { return n; }
 
-// 3-pass four-step schedule (N = 256).  Variant v: GA = y2 values per
-// outer-pass tile, GB = x group of the intermediate layout (runs of GA*GB
-// points), GM = x values per middle-pass tile (GM divides GB).
+// 3-pass four-step schedule (N = 128 and 256):
+// y = (N/16)*y1 + y2 with 16 values of y1 and N/16 of y2.  Variant v: GA = y2
+// values per outer-pass tile, GB = x group of the intermediate layout (runs
+// of GA*GB points), GM = x values per middle-pass tile (GM divides GB).
 constexpr int kSplitVariants = 9;
 constexpr int split_ga(int v) { return v == 2 || v == 3 ? 4 : 2; }
 constexpr int split_gb(int v) { return v == 0 || v == 2 ? 2 : v <= 5 ? 4 : 8; }
 constexpr int split_gm(int v) {
   return v == 0 || v == 2 || v == 4 || v == 7 ? 2 : v == 1 || v == 3 || v == 6 ? 4 : 1;
 }
-constexpr int split_a_threads(int v) { return 256 * split_ga(v); }
-constexpr int split_a_tiles(int v) { return 256 * (16 / split_ga(v)); }
-constexpr int split_b_threads(int v) { return 256 * split_gm(v); }
-constexpr int split_b_tiles(int v) { return 16 * (256 / split_gm(v)); }
+constexpr bool split_supported(int n, int v) { return (n == 128 || n == 256) && v >= 0 && v < kSplitVariants; }
+constexpr int split_a_threads(int n, int v) { return n * split_ga(v); }
+constexpr int split_a_tiles(int n, int v) { return n * ((n / 16) / split_ga(v)); }
+constexpr int split_b_threads(int n, int v) { return (n / 16) * split_gm(v) * (n / 16); }
+constexpr int split_b_tiles(int n, int v) { return 16 * (n / split_gm(v)); }
 
 }  // namespace cfgeo

@@ -97,49 +97,67 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
                   cfk::PlaneCfg<32>::THREADS == cfgeo::plane_threads(32),
               "plane geometry");
 
-#define CF_SPLIT_AC(GA, GB)                                                                                     \
-  extern "C" __global__ __launch_bounds__(256 * GA) void cf_xy1_fwd_256_g##GA##GB(CF_ARGS) {                    \
-    cfk::xy1_kernel<GA, GB, false>(in, out, tw, scale, ntotal);                                                 \
+#define CF_SPLIT_AC(N, GA, GB)                                                                                  \
+  extern "C" __global__ __launch_bounds__(N * GA) void cf_xy1_fwd_##N##_g##GA##GB(CF_ARGS) {                     \
+    cfk::xy1_kernel<N, GA, GB, false>(in, out, tw, scale, ntotal);                                              \
   }                                                                                                             \
-  extern "C" __global__ __launch_bounds__(256 * GA) void cf_xy1_inv_256_g##GA##GB(CF_ARGS) {                    \
-    cfk::xy1_kernel<GA, GB, true>(in, out, tw, scale, ntotal);                                                  \
+  extern "C" __global__ __launch_bounds__(N * GA) void cf_xy1_inv_##N##_g##GA##GB(CF_ARGS) {                     \
+    cfk::xy1_kernel<N, GA, GB, true>(in, out, tw, scale, ntotal);                                               \
   }
-#define CF_SPLIT_B(GA, GB, GM, SUFFIX)                                                                          \
-  extern "C" __global__ __launch_bounds__(256 * GM) void cf_mid_256_##SUFFIX(CF_ARGS) {                         \
-    cfk::mid_kernel<GA, GB, GM, (GM <= 2)>(in, out, H, tw, hstride, ntotal);                                    \
+#define CF_SPLIT_B(N, GA, GB, GM, SUFFIX)                                                                       \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_mid_##N##_##SUFFIX(CF_ARGS) { \
+    cfk::mid_kernel<N, GA, GB, GM, (GM <= 2)>(in, out, H, tw, hstride, ntotal);                                 \
   }                                                                                                             \
-  extern "C" __global__ __launch_bounds__(256) void cf_hlayout_256_##SUFFIX(CF_ARGS) {                          \
-    cfk::h_layout_body<GM>(in, out);                                                                            \
+  extern "C" __global__ __launch_bounds__(256) void cf_hlayout_##N##_##SUFFIX(CF_ARGS) {                        \
+    cfk::h_layout_body<N, GM>(in, out);                                                                         \
   }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
-// (the registers fit; measured 2-3% faster)
-CF_SPLIT_AC(2, 2)
-CF_SPLIT_AC(2, 4)
-CF_SPLIT_AC(4, 2)
-CF_SPLIT_AC(4, 4)
-CF_SPLIT_AC(2, 8)
-CF_SPLIT_B(2, 2, 2, g22)
-CF_SPLIT_B(2, 4, 4, g24)
-CF_SPLIT_B(4, 2, 2, g42)
-CF_SPLIT_B(4, 4, 4, g44)
-CF_SPLIT_B(2, 4, 2, g24m2)
-CF_SPLIT_B(2, 4, 1, g24m1)
-CF_SPLIT_B(2, 8, 4, g28m4)
-CF_SPLIT_B(2, 8, 2, g28m2)
-CF_SPLIT_B(2, 8, 1, g28m1)
-template <int V>
+// (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
+CF_SPLIT_AC(256, 2, 2)
+CF_SPLIT_AC(256, 2, 4)
+CF_SPLIT_AC(256, 4, 2)
+CF_SPLIT_AC(256, 4, 4)
+CF_SPLIT_AC(256, 2, 8)
+CF_SPLIT_B(256, 2, 2, 2, g22)
+CF_SPLIT_B(256, 2, 4, 4, g24)
+CF_SPLIT_B(256, 4, 2, 2, g42)
+CF_SPLIT_B(256, 4, 4, 4, g44)
+CF_SPLIT_B(256, 2, 4, 2, g24m2)
+CF_SPLIT_B(256, 2, 4, 1, g24m1)
+CF_SPLIT_B(256, 2, 8, 4, g28m4)
+CF_SPLIT_B(256, 2, 8, 2, g28m2)
+CF_SPLIT_B(256, 2, 8, 1, g28m1)
+CF_SPLIT_AC(128, 2, 2)
+CF_SPLIT_AC(128, 2, 4)
+CF_SPLIT_AC(128, 4, 2)
+CF_SPLIT_AC(128, 4, 4)
+CF_SPLIT_AC(128, 2, 8)
+CF_SPLIT_B(128, 2, 2, 2, g22)
+CF_SPLIT_B(128, 2, 4, 4, g24)
+CF_SPLIT_B(128, 4, 2, 2, g42)
+CF_SPLIT_B(128, 4, 4, 4, g44)
+CF_SPLIT_B(128, 2, 4, 2, g24m2)
+CF_SPLIT_B(128, 2, 4, 1, g24m1)
+CF_SPLIT_B(128, 2, 8, 4, g28m4)
+CF_SPLIT_B(128, 2, 8, 2, g28m2)
+CF_SPLIT_B(128, 2, 8, 1, g28m1)
+template <int N, int V>
 struct SplitGeoCheck {
-  using K = cfk::Split256<cfgeo::split_ga(V), cfgeo::split_gb(V)>;
-  using M = cfk::MidGeo<cfgeo::split_gb(V), cfgeo::split_gm(V)>;
-  static_assert(K::A_THREADS == cfgeo::split_a_threads(V) && K::A_TILES == cfgeo::split_a_tiles(V) &&
-                    M::THREADS == cfgeo::split_b_threads(V) && M::TILES == cfgeo::split_b_tiles(V),
+  using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;
+  using M = cfk::MidGeoN<N, cfgeo::split_gb(V), cfgeo::split_gm(V)>;
+  static_assert(K::A_THREADS == cfgeo::split_a_threads(N, V) && K::A_TILES == cfgeo::split_a_tiles(N, V) &&
+                    M::THREADS == cfgeo::split_b_threads(N, V) && M::TILES == cfgeo::split_b_tiles(N, V),
                 "split geometry");
   static constexpr bool ok = true;
 };
-static_assert(SplitGeoCheck<0>::ok && SplitGeoCheck<1>::ok && SplitGeoCheck<2>::ok && SplitGeoCheck<3>::ok &&
-                  SplitGeoCheck<4>::ok && SplitGeoCheck<5>::ok && SplitGeoCheck<6>::ok && SplitGeoCheck<7>::ok &&
-                  SplitGeoCheck<8>::ok && cfgeo::kSplitVariants == 9,
+static_assert(SplitGeoCheck<256, 0>::ok && SplitGeoCheck<256, 1>::ok && SplitGeoCheck<256, 2>::ok &&
+                  SplitGeoCheck<256, 3>::ok && SplitGeoCheck<256, 4>::ok && SplitGeoCheck<256, 5>::ok &&
+                  SplitGeoCheck<256, 6>::ok && SplitGeoCheck<256, 7>::ok && SplitGeoCheck<256, 8>::ok &&
+                  SplitGeoCheck<128, 0>::ok && SplitGeoCheck<128, 1>::ok && SplitGeoCheck<128, 2>::ok &&
+                  SplitGeoCheck<128, 3>::ok && SplitGeoCheck<128, 4>::ok && SplitGeoCheck<128, 5>::ok &&
+                  SplitGeoCheck<128, 6>::ok && SplitGeoCheck<128, 7>::ok && SplitGeoCheck<128, 8>::ok &&
+                  cfgeo::kSplitVariants == 9,
               "");
 
 // ---- split-FP16 MFMA form of the four-step passes (split_mfma.h) ----------------------

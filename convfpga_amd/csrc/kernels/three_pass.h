@@ -113,31 +113,38 @@ __device__ __forceinline__ void plane_kernel(const float2* in, float2* out, cons
 }
 
 // ---------------------------------------------------------------------------
-// N = 256: four-step split of y
+// N in {128, 256}: four-step split of y, y = Y2*y1 + y2 with y1 in [0, 16) and
+// y2 in [0, Y2 = N/16); every line (x row or z line) is held by T = N/16
+// threads of 16 points
 // ---------------------------------------------------------------------------
-template <int GA, int GB>
-struct Split256 {
-  static constexpr int N = 256;
-  static constexpr int NXH = N / GB;               // x blocks
-  static constexpr int A_THREADS = 256 * GA;       // 16*GA rows of 16 threads (launch bound of xy1_*)
-  static constexpr int A_TILES = N * (16 / GA);    // (z, y2 group) per cube
-  static constexpr int A_ROW_PITCH = 272;          // 256 + 256/16
+template <int N_, int GA, int GB>
+struct SplitGeo {
+  static constexpr int N = N_;
+  static constexpr int Y2 = N / 16;               // y2 values: the middle pass's DFT size
+  static constexpr int T = N / 16;                // threads per x row
+  static_assert(Y2 % GA == 0 && N % GB == 0, "tile groups must divide the axes");
+  static constexpr int NXH = N / GB;              // x blocks
+  static constexpr int A_THREADS = 16 * GA * T;   // 16*GA rows of T threads (launch bound of xy1_*)
+  static constexpr int A_TILES = N * (Y2 / GA);   // (z, y2 group) per cube
+  static constexpr int A_ROW_PITCH = N + N / 16;
   // (kx, y1) block of one y2: kx stride 17, planes offset so that the two y2
   // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
-  static constexpr int A_COL_PLANE = 256 * 17 + (GA == 2 ? 1 : 4);
+  static constexpr int A_COL_PLANE = N * 17 + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
   // intermediate index of (x, k1, y2, z)
   __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
-    return (((k1 * NXH + x / GB) * 256 + z) * 16 + y2) * GB + (x % GB);
+    return (((k1 * NXH + x / GB) * N + z) * Y2 + y2) * GB + (x % GB);
   }
 };
+template <int GA, int GB>
+using Split256 = SplitGeo<256, GA, GB>;
 
 // INV = false: pass A (natural -> intermediate).  INV = true: pass C.
-template <int GA, int GB, bool INV>
+template <int N, int GA, int GB, bool INV>
 struct Xy1Pass {
-  using K = Split256<GA, GB>;
-  static constexpr int N = 256;
-  static constexpr int K1_STRIDE = K::NXH * 256 * 16 * GB;  // mid_index step of k1
+  using K = SplitGeo<N, GA, GB>;
+  static constexpr int Y2 = K::Y2, T = K::T;
+  static constexpr int K1_STRIDE = K::NXH * N * Y2 * GB;  // mid_index step of k1
   struct Regs {
     float2 v[16];
   };
@@ -146,23 +153,23 @@ struct Xy1Pass {
   const float2* tw_s;
   float2* lds;
   float scale;
-  int rt, row, ry1, ry2l;  // row identity: row = y2l*16 + y1, thread t holds x = t + 16*m
+  int rt, row, ry1, ry2l;  // row identity: row = y2l*16 + y1, thread t holds x = t + T*m
   int ckx, cy2l;           // column identity: (x, y2l) with x%GB fastest, then y2l, then x/GB
 
   __device__ __forceinline__ void tile_pos(int tile, size_t& cube, int& z, int& y2_0) const {
     cube = (size_t)(tile / K::A_TILES) * N * N * N;
     const int tt = tile % K::A_TILES;
-    z = tt / (16 / GA);
-    y2_0 = (tt % (16 / GA)) * GA;
+    z = tt / (Y2 / GA);
+    y2_0 = (tt % (Y2 / GA)) * GA;
   }
   __device__ __forceinline__ void load(int tile, Regs& r) const {
     size_t cube;
     int z, y2_0;
     tile_pos(tile, cube, z, y2_0);
     if constexpr (!INV) {
-      const float2* src = in + cube + (size_t)z * N * N + (16 * ry1 + y2_0 + ry2l) * N + rt;
+      const float2* src = in + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) r.v[m] = src[16 * m];
+      for (int m = 0; m < 16; ++m) r.v[m] = src[T * m];
     } else {
       const float2* src = in + cube + K::mid_index(ckx, 0, y2_0 + cy2l, z);
 #pragma unroll
@@ -178,7 +185,7 @@ struct Xy1Pass {
     if constexpr (!INV) {
       fft_line<N, false>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
 #pragma unroll
-      for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + 16 * m) * 17 + ry1] = r.v[m];
+      for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1] = r.v[m];
       __syncthreads();
       float2 u[16];
 #pragma unroll
@@ -198,12 +205,12 @@ struct Xy1Pass {
       for (int j = 0; j < 16; ++j) colp[j] = r.v[j];
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + 16 * m) * 17 + ry1];
+      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1];
       __syncthreads();
       fft_line<N, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
-      float2* dst = out + cube + (size_t)z * N * N + (16 * ry1 + y2_0 + ry2l) * N + rt;
+      float2* dst = out + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) dst[16 * m] = cscale(r.v[m], scale);
+      for (int m = 0; m < 16; ++m) dst[T * m] = cscale(r.v[m], scale);
     }
   }
 };
@@ -224,50 +231,57 @@ __device__ __forceinline__ int xcd_group_tile(int b) {
   }
 }
 
-template <int GA, int GB, bool INV>
+template <int N, int GA, int GB, bool INV>
 __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
-  using K = Split256<GA, GB>;
+  using K = SplitGeo<N, GA, GB>;
   constexpr int S = (GA * GB >= 16) ? 1 : 16 / (GA * GB);
   static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
   __shared__ float2 lds[K::A_LDS];
-  __shared__ float2 tw_s[256];
-  stage_twiddles<256, K::A_THREADS>(tw_s, tw);
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, K::A_THREADS>(tw_s, tw);
   const int x = threadIdx.x;
-  const int row = x / 16;
+  const int row = x / K::T;
   const int cxl = x % GB, cy2l = (x / GB) % GA, cxh = x / (GB * GA);
-  const Xy1Pass<GA, GB, INV> k{in, out, tw_s, lds, scale, x % 16, row, row % 16, row / 16, cxh * GB + cxl, cy2l};
+  const Xy1Pass<N, GA, GB, INV> k{in, out, tw_s, lds, scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l};
   const int tile = xcd_group_tile<S>((int)blockIdx.x);
   if (tile >= ntotal) return;
-  typename Xy1Pass<GA, GB, INV>::Regs a;
+  typename Xy1Pass<N, GA, GB, INV>::Regs a;
   k.load(tile, a);
   k.process(tile, a);
 }
 
 // Middle pass geometry: tile = (k1, GM adjacent x) of a layout with x-group GB.
-template <int GB, int GM>
-struct MidGeo {
+template <int N, int GB, int GM>
+struct MidGeoN {
   static_assert(GB % GM == 0, "the mid tile width must divide the layout's x group");
-  static constexpr int NXM = 256 / GM;          // mid tiles per k1
-  static constexpr int THREADS = 256 * GM;      // 16*GM z-lines of 16 threads (launch bound of mid)
+  static constexpr int Y2 = N / 16;
+  static constexpr int T = N / 16;              // threads per z-line
+  static constexpr int NXM = N / GM;            // mid tiles per k1
+  static constexpr int LINES = Y2 * GM;         // z-lines per tile: (y2, x%GM)
+  static constexpr int THREADS = LINES * T;     // launch bound of mid
   static constexpr int TILES = 16 * NXM;        // (k1, x block) per cube
-  static constexpr int LINES = 16 * GM;         // z-lines per tile: (y2, x%GM)
-  // row pitch 17*GM (= GM mod 32): the (kz, x%GM) reads of the y2 transpose
-  // step 2*GM banks per kz, so a half-wave touches every bank pair once
-  static constexpr int PITCH = 17 * GM;
-  static constexpr int LDS = 256 * PITCH;
-  static constexpr int HTILE = 4096 * GM;       // spectrum points per tile
+  static constexpr int ROWS = N * GM;           // (kz, x%GM) rows of the y2 transform
+  static constexpr int RPT = ROWS / THREADS;    // rows per thread (1 at 256, 2 at 128)
+  static_assert(RPT * THREADS == ROWS, "rows must split evenly over the threads");
+  // row pitch (Y2+1)*GM (17*GM at 256 = GM mod 32): the (kz, x%GM) reads of
+  // the y2 transform step 2*GM banks per kz
+  static constexpr int PITCH = (Y2 + 1) * GM;
+  static constexpr int LDS = N * PITCH;
+  static constexpr int HTILE = N * Y2 * GM;     // spectrum points per tile
   static constexpr int SIBLINGS = GB / GM;      // tiles sharing the layout's x group
 };
+template <int GB, int GM>
+using MidGeo = MidGeoN<256, GB, GM>;
 
 // Middle pass on tile (k1, x0 .. x0+GM-1).
-template <int GA, int GB, int GM, bool HPF = false>
+template <int N, int GA, int GB, int GM, bool HPF = false>
 struct MidPass {
-  using K = Split256<GA, GB>;
-  using M = MidGeo<GB, GM>;
-  static constexpr int N = 256, L = M::LINES;
+  using K = SplitGeo<N, GA, GB>;
+  using M = MidGeoN<N, GB, GM>;
+  static constexpr int L = M::LINES, Y2 = M::Y2, T = M::T, RPT = M::RPT;
   struct Regs {
     float2 v[16];
-    float2 hv[HPF ? 16 : 1];  // HPF: the spectrum values, loaded with the points
+    float2 hv[HPF ? RPT * Y2 : 1];  // HPF: the spectrum values, loaded with the points
   };
   const float2* in;
   float2* out;
@@ -275,96 +289,109 @@ struct MidPass {
   const float2* tw_s;
   float2* lds;
   long long hstride;
-  int l, zt;  // z-line identity: line l = y2*GM + x%GM, thread zt holds z = zt + 16*m
-  int qx, qz; // (kz, x%GM) identity for the y2 transform
+  int l, zt;  // z-line identity: line l = y2*GM + x%GM, thread zt holds z = zt + T*m
+  int q;      // first (kz, x%GM) row of the y2 transform: rows q + THREADS*rr, row = kz*GM + x%GM
 
-  // offset of this thread's point z = zt (m = 0); point m is 16*16*GB*m further
+  // offset of this thread's point z = zt (m = 0); point m is T*Y2*GB*m further
   __device__ __forceinline__ size_t thread_off(int tile) const {
     const int tt = tile % M::TILES;
     const int x0 = (tt % M::NXM) * GM;
     const size_t cube = (size_t)(tile / M::TILES) * N * N * N;
     return cube + K::mid_index(x0 + l % GM, tt / M::NXM, l / GM, zt);
   }
-  __device__ __forceinline__ const float2* h_ptr(int tile) const {
-    return H + (size_t)(tile / M::TILES) * hstride + (size_t)(tile % M::TILES) * M::HTILE + qz * GM + qx;
+  // spectrum values of row `row` of the tile: [k1][x/GM][k2][kz][x%GM], k2 stride N*GM
+  __device__ __forceinline__ const float2* h_ptr(int tile, int row) const {
+    return H + (size_t)(tile / M::TILES) * hstride + (size_t)(tile % M::TILES) * M::HTILE + row;
   }
   __device__ __forceinline__ void load(int tile, Regs& r) const {
     const float2* src = in + thread_off(tile);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) r.v[m] = src[256 * GB * m];
+    for (int m = 0; m < 16; ++m) r.v[m] = src[T * Y2 * GB * m];
     if constexpr (HPF) {
-      const float2* h = h_ptr(tile);
 #pragma unroll
-      for (int k2 = 0; k2 < 16; ++k2) r.hv[k2] = h[k2 * 256 * GM];
+      for (int rr = 0; rr < RPT; ++rr) {
+        const float2* h = h_ptr(tile, q + M::THREADS * rr);
+#pragma unroll
+        for (int k2 = 0; k2 < Y2; ++k2) r.hv[rr * Y2 + k2] = h[k2 * N * GM];
+      }
     }
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
     constexpr int PITCH = M::PITCH;
     const PitchLds<PITCH> view{lds + l};
-    fft_line<N, false>(r.v, zt, view, tw_s);  // r.v[m] = Z(kz = zt + 16 m) of line l
+    fft_line<N, false>(r.v, zt, view, tw_s);  // r.v[m] = Z(kz = zt + T m) of line l
 #pragma unroll
-    for (int m = 0; m < 16; ++m) lds[(zt + 16 * m) * PITCH + l] = r.v[m];
+    for (int m = 0; m < 16; ++m) lds[(zt + T * m) * PITCH + l] = r.v[m];
     __syncthreads();
-    float2 u[16];
-    float2* rowp = lds + qz * PITCH + qx;
+    float2 u[RPT][Y2];
 #pragma unroll
-    for (int y2 = 0; y2 < 16; ++y2) u[y2] = rowp[y2 * GM];
-    float2 hv[16];
-    if constexpr (HPF) {
+    for (int rr = 0; rr < RPT; ++rr) {
+      const int row = q + M::THREADS * rr;
+      float2* rowp = lds + (row / GM) * PITCH + row % GM;
 #pragma unroll
-      for (int k2 = 0; k2 < 16; ++k2) hv[k2] = r.hv[k2];
-    } else {
-      const float2* h = h_ptr(tile);
+      for (int y2 = 0; y2 < Y2; ++y2) u[rr][y2] = rowp[y2 * GM];
+      float2 hv[Y2];
+      if constexpr (HPF) {
 #pragma unroll
-      for (int k2 = 0; k2 < 16; ++k2) hv[k2] = h[k2 * 256 * GM];
+        for (int k2 = 0; k2 < Y2; ++k2) hv[k2] = r.hv[rr * Y2 + k2];
+      } else {
+        const float2* h = h_ptr(tile, row);
+#pragma unroll
+        for (int k2 = 0; k2 < Y2; ++k2) hv[k2] = h[k2 * N * GM];
+      }
+      DFT<Y2, false>::run(u[rr]);  // u[k2]: full 3D spectrum at (x, k1 + 16 k2, kz)
+#pragma unroll
+      for (int k2 = 0; k2 < Y2; ++k2) u[rr][k2] = cmul(u[rr][k2], hv[k2]);
+      DFT<Y2, true>::run(u[rr]);  // u[y2]
     }
-    DFT<16, false>::run(u);  // u[k2]: full 3D spectrum at (x, k1 + 16 k2, qz)
+    __syncthreads();  // every thread has read its rows before they are overwritten
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) u[k2] = cmul(u[k2], hv[k2]);
-    DFT<16, true>::run(u);  // u[y2]
-    __syncthreads();  // every thread has read its row before it is overwritten
+    for (int rr = 0; rr < RPT; ++rr) {
+      const int row = q + M::THREADS * rr;
+      float2* rowp = lds + (row / GM) * PITCH + row % GM;
 #pragma unroll
-    for (int y2 = 0; y2 < 16; ++y2) rowp[y2 * GM] = u[y2];
+      for (int y2 = 0; y2 < Y2; ++y2) rowp[y2 * GM] = u[rr][y2];
+    }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + 16 * m) * PITCH + l];
+    for (int m = 0; m < 16; ++m) r.v[m] = lds[(zt + T * m) * PITCH + l];
     __syncthreads();
     fft_line<N, true>(r.v, zt, view, tw_s);
     float2* dst = out + thread_off(tile);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) dst[256 * GB * m] = r.v[m];
+    for (int m = 0; m < 16; ++m) dst[T * Y2 * GB * m] = r.v[m];
   }
 };
 
-template <int GA, int GB, int GM, bool HPF = false>
+template <int N, int GA, int GB, int GM, bool HPF = false>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
-  using M = MidGeo<GB, GM>;
+  using M = MidGeoN<N, GB, GM>;
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
-  __shared__ float2 tw_s[256];
-  stage_twiddles<256, M::THREADS>(tw_s, tw);
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, M::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
-  const MidPass<GA, GB, GM, HPF> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x % GM, x / GM};
+  const MidPass<N, GA, GB, GM, HPF> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x};
   // siblings (consecutive tiles of one x group) on one XCD, back to back
   const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
   if (tile >= ntotal) return;
-  typename MidPass<GA, GB, GM, HPF>::Regs a;
+  typename MidPass<N, GA, GB, GM, HPF>::Regs a;
   k.load(tile, a);
   k.process(tile, a);
 }
 
 // Spectrum re-layout for the middle pass: natural FFT3 order -> [k1][x/GM][k2][kz][x%GM].
-template <int GM>
+template <int N, int GM>
 __device__ __forceinline__ void h_layout_body(const float2* __restrict__ in, float2* __restrict__ out) {
-  constexpr int N = 256, NXM = N / GM;
+  constexpr int NXM = N / GM, Y2 = N / 16;
   const int i = blockIdx.x * 256 + threadIdx.x;  // output index within the cube
   const size_t cube = (size_t)blockIdx.y * N * N * N;
   const int xl = i % GM;
-  const int kz = (i / GM) % 256;
-  const int k2 = (i / (GM * 256)) % 16;
-  const int xh = (i / (GM * 256 * 16)) % NXM;
-  const int k1 = i / (GM * 256 * 16 * NXM);
+  const int kz = (i / GM) % N;
+  const int k2 = (i / (GM * N)) % Y2;
+  const int xh = (i / (GM * N * Y2)) % NXM;
+  const int k1 = i / (GM * N * Y2 * NXM);
   const int x = xh * GM + xl, ky = k1 + 16 * k2;
   out[cube + i] = in[cube + ((size_t)kz * N + ky) * N + x];
 }

@@ -43,7 +43,7 @@ extern "C" {
 #define CF_PLAN_AUTO 0       /* the autotuner's choice (default)                              */
 #define CF_PLAN_AXIS5 1      /* x, y, z*H*z, y^-1, x^-1: 11 volume transfers, every N          */
 #define CF_PLAN_PLANE3 2     /* xy-plane, z*H*z, xy-plane^-1: 7 transfers, N in {32,64,128}    */
-#define CF_PLAN_SPLIT3_G22 3 /* four-step y split, 7 transfers, N = 256, tile groups 2x2       */
+#define CF_PLAN_SPLIT3_G22 3 /* four-step y split, 7 transfers, N = 128, 256, tile groups 2x2  */
 #define CF_PLAN_SPLIT3_G24 4 /*   ... tile groups 2x4                                          */
 #define CF_PLAN_SPLIT3_G42 5 /*   ... tile groups 4x2                                          */
 #define CF_PLAN_SPLIT3_G44 6 /*   ... tile groups 4x4                                          */

@@ -118,7 +118,8 @@ def test_plan_support_table():
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5) == 1
         assert L.cf_plan_supported(n, api.CF_PLAN_PLANE3) == (1 if n in (32, 64, 128) else 0)
         for p in range(api.CF_PLAN_SPLIT3_G22, api.CF_PLAN_SPLIT3_G28M1 + 1):
-            assert L.cf_plan_supported(n, p) == (1 if n == 256 else 0)
+            want = n in (128, 256)
+            assert L.cf_plan_supported(n, p) == (1 if want else 0)
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5Z) == (1 if n >= 64 else 0)
     assert api.CF_NUM_PLANS == api.CF_PLAN_AXIS5Z + 1
     assert L.cf_plan_supported(1024, api.CF_PLAN_AXIS5) == 0

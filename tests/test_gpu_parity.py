@@ -267,7 +267,7 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
-    [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)]
+    [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
@@ -355,18 +355,20 @@ def test_autotuner_picks_a_supported_plan(gpu):
         assert api.get_plan(n) == plan
 
 
-def _mixed_instances(gpu, oracle, sizes, plan_for_256):
-    """(sig, out, spectrum, host sig, host filt) per instance, each with its own filter."""
+def _mixed_instances(gpu, oracle, sizes, plans):
+    """(sig, out, spectrum, host sig, host filt) per instance, each with its own
+    filter; plans maps an edge to the plan its spectrum is made for (others:
+    the autotuned plan)."""
     api = gpu.api
     inst = []
     for k, n in enumerate(sizes):
-        if n == 256:
-            api.set_plan(256, plan_for_256)
+        if n in plans:
+            api.set_plan(n, plans[n])
         s = oracle.uniform(n ** 3, 50000 + 10 * k)
         f = oracle.uniform(n ** 3, 60000 + 10 * k)
         spec = gpu.Spectrum(n, _t(f))
-        if n == 256:
-            api.set_plan(256, api.CF_PLAN_AUTO)
+        if n in plans:
+            api.set_plan(n, api.CF_PLAN_AUTO)
         ds = _t(s)
         inst.append((ds, ds.clone().zero_(), spec, s, f))
     return inst
@@ -389,7 +391,9 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
     api = gpu.api
     sizes = [32, 64, 16, 128, 32, 256, 64, 32]
     print("case batch mode=%d plan256=%d sizes=%s" % (mode, plan256, sizes), flush=True)
-    inst = _mixed_instances(gpu, oracle, sizes, plan256)
+    # the persistent kernel reads spectra in natural order
+    plans = {256: plan256, 128: api.CF_PLAN_AXIS5} if mode == api.CF_BATCH_PERSISTENT else {256: plan256}
+    inst = _mixed_instances(gpu, oracle, sizes, plans)
     b = gpu.api.Batch([(ds, do, spec) for ds, do, spec, _, _ in inst], mode=mode)
     for rep in range(3):
         for _, do, _, _, _ in inst:
@@ -421,7 +425,10 @@ def test_batch_spectrum_views_grouped(gpu, oracle, mode):
         host[n] = (s, f)
         dsig = _t(np.concatenate(s))
         dev[n] = (dsig, torch.zeros_like(dsig))
+        if mode == api.CF_BATCH_PERSISTENT and n >= 128:  # the persistent kernel reads natural-order spectra
+            api.set_plan(n, api.CF_PLAN_AXIS5)
         parent = gpu.Spectrum(n, _t(np.concatenate(f)), count=c)
+        api.set_plan(n, api.CF_PLAN_AUTO)
         specs[n] = [parent.view(k) for k in range(c)]
         del parent  # the views keep it alive
     order = [(32, 0), (32, 1), (64, 0), (64, 1), (32, 2), (128, 0), (128, 1)]
