@@ -1585,8 +1585,6 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
       if (needs_work(x.h->plan))
         return fail(CF_ERR_ARG, "instance %u: spectrum in the private layout of a four-step or z-blocked plan; "
                     "create it under CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
-    } else if (needs_work(x.h->plan)) {
-      work_need = std::max(work_need, cube_bytes(x.h->n));
     }
   }
   if (!g.capture) HIPCHK(hipStreamCreateWithFlags(&g.capture, hipStreamNonBlocking));
@@ -1654,6 +1652,32 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
       rc = fail(CF_ERR_RUNTIME, "batch kernel capture failed");
     rc = batch_capture_end(b, rc);
   } else {
+    // runs of consecutive instances with one edge and plan and contiguous
+    // signals / outputs, and either one spectrum or spectra that follow each
+    // other in memory (views of one multi-filter spectrum), are recorded as
+    // one batched pass sequence; plans that go through the workspace get
+    // room for their longest run
+    struct Run {
+      unsigned i, run;
+      bool shared;
+    };
+    std::vector<Run> runs;
+    work_need = 0;
+    for (unsigned i = 0; i < count;) {
+      const unsigned n = inst[i].h->n;
+      const size_t V = cube_bytes(n);
+      unsigned run = 1;
+      const bool shared = i + 1 < count && inst[i + 1].h == inst[i].h;
+      while (i + run < count && inst[i + run].h->n == n && inst[i + run].h->plan == inst[i].h->plan &&
+             (shared ? inst[i + run].h == inst[i].h
+                     : (const char*)inst[i + run].h->d == (const char*)inst[i].h->d + run * V) &&
+             (const char*)inst[i + run].d_sig == (const char*)inst[i].d_sig + run * V &&
+             (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V)
+        ++run;
+      if (needs_work(inst[i].h->plan)) work_need = std::max(work_need, (size_t)run * V);
+      runs.push_back({i, run, shared});
+      i += run;
+    }
     if (work_need) {
       const hipError_t e = hipMalloc(&b->d_work, work_need);
       if (e != hipSuccess) {
@@ -1662,29 +1686,15 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
       }
     }
     HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
-    // runs of consecutive instances with one edge and contiguous signals /
-    // outputs, and either one spectrum or spectra that follow each other in
-    // memory (views of one multi-filter spectrum), are recorded as one batched
-    // pass sequence
-    for (unsigned i = 0; i < count && rc == CF_OK;) {
-      const unsigned n = inst[i].h->n;
-      const size_t V = cube_bytes(n);
-      const bool splitp = needs_work(inst[i].h->plan);
-      unsigned run = 1;
-      const bool shared = i + 1 < count && inst[i + 1].h == inst[i].h;
-      while (i + run < count && !splitp && inst[i + run].h->n == n && inst[i + run].h->plan == inst[i].h->plan &&
-             (shared ? inst[i + run].h == inst[i].h
-                     : (const char*)inst[i + run].h->d == (const char*)inst[i].h->d + run * V) &&
-             (const char*)inst[i + run].d_sig == (const char*)inst[i].d_sig + run * V &&
-             (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V)
-        ++run;
-      const long long hstride = (run > 1 && !shared) ? (long long)cube_elems(n) : 0;
+    for (const Run& r : runs) {
+      if (rc != CF_OK) break;
+      const unsigned i = r.i, n = inst[i].h->n;
+      const long long hstride = (r.run > 1 && !r.shared) ? (long long)cube_elems(n) : 0;
       const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
       std::vector<Pass> p;
-      conv_passes(*fns[i], inst[i].h->plan, n, run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, hstride,
+      conv_passes(*fns[i], inst[i].h->plan, n, r.run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, hstride,
                   scale, p);
-      rc = run_passes(p, n, run, g.capture, fns[i]->tw, nullptr);
-      i += run;
+      rc = run_passes(p, n, r.run, g.capture, fns[i]->tw, nullptr);
     }
     rc = batch_capture_end(b, rc);
   }

@@ -407,8 +407,8 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
     b.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batch_spectrum_views_grouped(gpu, oracle, mode):
+@pytest.mark.parametrize("mode,plan128", [(0, 0), (1, 0), (1, 12)])
+def test_batch_spectrum_views_grouped(gpu, oracle, mode, plan128):
     """Instances whose spectra are consecutive views of one multi-filter
     spectrum and whose signals / outputs are consecutive slices: the graph
     batch records each such run as one batched pass sequence (own filter per
@@ -416,7 +416,7 @@ def test_batch_spectrum_views_grouped(gpu, oracle, mode):
     import torch
     api = gpu.api
     counts = {32: 3, 64: 2, 128: 2}
-    print("case batch spectrum views mode=%d counts=%s" % (mode, counts), flush=True)
+    print("case batch spectrum views mode=%d plan128=%d counts=%s" % (mode, plan128, counts), flush=True)
     host, dev, specs = {}, {}, {}
     for n, c in counts.items():
         P = n ** 3
@@ -427,6 +427,8 @@ def test_batch_spectrum_views_grouped(gpu, oracle, mode):
         dev[n] = (dsig, torch.zeros_like(dsig))
         if mode == api.CF_BATCH_PERSISTENT and n >= 128:  # the persistent kernel reads natural-order spectra
             api.set_plan(n, api.CF_PLAN_AXIS5)
+        elif n == 128 and plan128:  # runs of a plan that goes through the workspace, batched
+            api.set_plan(n, plan128)
         parent = gpu.Spectrum(n, _t(np.concatenate(f)), count=c)
         api.set_plan(n, api.CF_PLAN_AUTO)
         specs[n] = [parent.view(k) for k in range(c)]
