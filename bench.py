@@ -14,7 +14,9 @@ launch stream during the timed steps; the roofline entry is for the pass with
 the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
 convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
-against the FP64 oracle.
+against the FP64 oracle.  Beside the FP32 form the line carries the same
+convolution in the split-FP16 MFMA form (its time, its speed relative to FP32
+and its rel-L2 against the FP64 oracle), the second half of the metric.
 
 Other workloads (reported lines, not the metric): c2_128 / c1_64 / c5_512 (one cube),
 c3_64x64 (64 cubes of 64^3, one shared spectrum), c4_mixed (16x32^3 + 8x64^3 +
@@ -63,7 +65,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
+def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps, split_out_host=None):
     """Times the oracle's FP32 CPU convolution (port of src/fft_conv3D.cpp:50-177)
     on `reps` convolutions of one cube, and checks the GPU output on the same
     inputs.  With split_check_leg (the c5 checker), the only places bench.py
@@ -103,6 +105,8 @@ def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps):
         check["snr_db_vs_fp32_cpu"] = O.snr_db(np.ascontiguousarray(ref32), first)
         ref64 = O.conv3d_f64(n, sig, filt, 1.0 / P)
         check["rel_l2_vs_fp64"] = O.rel_l2(first, ref64)
+        if split_out_host is not None:
+            check["rel_l2_vs_fp64_split_fp16_mfma"] = O.rel_l2(np.ascontiguousarray(split_out_host[:P]), ref64)
     return res, check
 
 
@@ -205,7 +209,9 @@ def run_single(a, torch, C, api):
     tfile = os.path.join(ROOT, "profiles", "traffic_256.json")
     if n == 256 and os.path.exists(tfile):
         traffic = json.load(open(tfile)).get(kname, {}).get("traffic_bytes")
-    line = base_line(a, "3D convs/sec + %% HBM roofline at %d^3 c2c" % n, a.steps / wall, "conv/s",
+    metric = ("3D convs/sec + % HBM roofline at 256\u00b3 c2c; split-MFMA vs FP32 at rel-L2\u22641e-5" if n == 256
+              else "3D convs/sec + %% HBM roofline at %d^3 c2c" % n)
+    line = base_line(a, metric, a.steps / wall, "conv/s",
                      1e3 * wall / a.steps,
                      {"workload": "conv3d_%d^3_c2c_fp32_batch1_precomputed_filter_spectrum" % n, "N": n, "batch": 1,
                       "plan": api.PLAN_NAMES[plan], "passes": len(passes), "scale": "1/N^3",
@@ -222,13 +228,40 @@ def run_single(a, torch, C, api):
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
     line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, out)
     line["slab_share"] = share_leg(C, n, spec, sig, out)
+    split_host = None
+    if n == 256:
+        line["split_fp16_mfma"], split_host = split_leg(a, torch, C, api, n, spec, sig, filt, wall_plain / a.steps)
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
         line["host_batch"] = host_batch_leg(torch, C, n, spec, 4)
     if a.power:
         line["power"] = power_leg(torch, C, spec, sig, out)
-    return line, (n, host_out, seed_sig, seed_filt)
+    return line, (n, host_out, seed_sig, seed_filt, split_host)
+
+
+def split_leg(a, torch, C, api, n, spec, sig, filt, fp32_s):
+    """The same convolution in the split-FP16 MFMA form (radix-16 DFTs as
+    hi/lo FP16 complex GEMMs with FP32 accumulation), timed like the FP32
+    loop without events; its output is checked against the FP64 oracle in the
+    cpu_baseline leg."""
+    sp = spec
+    if spec.plan != api.CF_PLAN_SPLIT3_G24M2:  # the MFMA kernels read this plan's spectrum layout
+        api.set_plan(n, api.CF_PLAN_SPLIT3_G24M2)
+        sp = C.Spectrum(n, filt)
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    out2 = torch.empty_like(sig)
+    for _ in range(a.warmup):
+        C.conv3d_device(sp, sig, out2, precision=api.CF_PREC_SPLIT_FP16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        C.conv3d_device(sp, sig, out2, precision=api.CF_PREC_SPLIT_FP16)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    res = {"ms_per_conv": 1e3 * t, "conv_per_s": 1.0 / t, "speed_vs_fp32": fp32_s / t,
+           "fp32_ms_per_conv_same_loop": 1e3 * fp32_s}
+    return res, (out2.cpu().numpy() if not a.no_cpu_baseline else None)
 
 
 def power_leg(torch, C, spec, sig, out, seconds=2.0, idle_s=0.5):
@@ -603,7 +636,8 @@ def main():
         line, cpu = (run_single if a.workload in SINGLE else run_batch)(a, torch, C, api)
         line["cpu_baseline"] = None
         if cpu is not None and not a.no_cpu_baseline:
-            base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps)
+            base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps,
+                                           cpu[4] if len(cpu) > 4 else None)
             line["cpu_baseline"] = base
             line.update(check)
     C.fpga_final()
