@@ -226,12 +226,14 @@ def run_single(a, torch, C, api):
     line["copy_kernel_GBps"] = copy_gbs
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
-    line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, out)
-    line["slab_share"] = share_leg(C, n, spec, sig, out)
+    # the timed convolution's output, checked against the oracle in the cpu_baseline leg
+    host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
+    scratch = torch.empty_like(out)  # the legs below write their own outputs here
+    line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, scratch)
+    line["slab_share"] = share_leg(C, n, spec, sig, scratch)
     split_host = None
     if n == 256:
         line["split_fp16_mfma"], split_host = split_leg(a, torch, C, api, n, spec, sig, filt, wall_plain / a.steps)
-    host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     if a.host_calls:
         line["dropin_host_calls"] = dropin_host_leg(C, n, sig, filt)
         line["host_batch"] = host_batch_leg(torch, C, n, spec, 4)
