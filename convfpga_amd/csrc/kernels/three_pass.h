@@ -6,18 +6,19 @@
 //   colz_conv : z forward, product with the spectrum, z inverse (axis pass kernel)
 //   plane_inv : inverse 2D FFT of one xy-plane (y then x) with the final scale
 //
-// N = 256 (a plane does not fit): the y transform is split four-step style,
-// y = 16*y1 + y2, ky = k1 + 16*k2:
-//   X[k1 + 16 k2] = sum_y2 W16^(y2 k2) W256^(y2 k1) [ sum_y1 x[16 y1 + y2] W16^(y1 k1) ]
-//   xy1_fwd : x FFT of 16*GA rows, 16-point DFT over y1, twiddle W256^(y2 k1)
-//   mid_conv: z FFT, 16-point DFT over y2, product with the spectrum, inverse
-//             16-point DFT over k2, inverse z FFT -- one contiguous tile
-//   xy1_inv : twiddle W256^(-y2 k1), inverse 16-point DFT over k1, inverse x
+// N = 128 and 256 (also the four-step form at 128, where a plane tile gives
+// too few work-groups): the y transform is split four-step style with
+// Y2 = N/16, y = Y2*y1 + y2 (16 values of y1), ky = k1 + 16*k2:
+//   X[k1 + 16 k2] = sum_y2 W_Y2^(y2 k2) W_N^(y2 k1) [ sum_y1 x[Y2 y1 + y2] W16^(y1 k1) ]
+//   xy1_fwd : x FFT of 16*GA rows, 16-point DFT over y1, twiddle W_N^(y2 k1)
+//   mid_conv: z FFT, Y2-point DFT over y2, product with the spectrum, inverse
+//             Y2-point DFT over k2, inverse z FFT -- one contiguous tile
+//   xy1_inv : twiddle W_N^(-y2 k1), inverse 16-point DFT over k1, inverse x
 //             FFT, final scale, natural order out
 // The intermediate between the passes (in the workspace) is indexed by
 // (x, k1, y2, z) and laid out [k1][x/GB][z][y2][x%GB]: xy1_* move runs of
 // GA*GB contiguous points.  A mid_conv tile is (k1, GM adjacent x) with GM
-// dividing GB: for GM = GB one contiguous block of 4096*GB points, for GM < GB
+// dividing GB: for GM = GB one contiguous block of N*Y2*GB points, for GM < GB
 // runs of GM points whose sibling tiles (same k1, same x/GB) run on the same
 // XCD.  The filter spectrum for mid_conv is stored [k1][x/GM][k2][kz][x%GM] so
 // that each tile reads its own contiguous block.
