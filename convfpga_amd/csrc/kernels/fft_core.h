@@ -40,6 +40,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
+// load marked non-temporal (read once: it should not displace data that is
+// read again, such as a pass's intermediate kept in the MALL)
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  return __builtin_bit_cast(float2, __builtin_nontemporal_load(reinterpret_cast<const double*>(p)));
+}
+
 // cos / sin of 2*pi*j/16, j = 0..7 (correctly rounded float literals)
 struct Tab16 {
   static constexpr float c[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
