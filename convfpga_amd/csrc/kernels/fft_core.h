@@ -46,6 +46,24 @@ __device__ __forceinline__ float2 ld_stream(const float2* p) {
   return __builtin_bit_cast(float2, __builtin_nontemporal_load(reinterpret_cast<const double*>(p)));
 }
 
+// Buffer accesses of one cube (at most 2^31 bytes): a per-thread 32-bit byte
+// offset plus a wave-uniform one, instead of a 64-bit address per access.
+typedef int i2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buffer_ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+// the same load marked non-temporal (cache-policy bit 1: read once); being an
+// intrinsic call it is never merged with an ordinary load of the same address
+__device__ __forceinline__ float2 buffer_ld2_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2));
+}
+__device__ __forceinline__ void buffer_st2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2_t, v), r, (int)voff, (int)soff, 0);
+}
+
 // cos / sin of 2*pi*j/16, j = 0..7 (correctly rounded float literals)
 struct Tab16 {
   static constexpr float c[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,

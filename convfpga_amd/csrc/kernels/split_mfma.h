@@ -185,18 +185,6 @@ __device__ __forceinline__ void lds_get(const float2* lds, int p, float& re, flo
   im = f[1];
 }
 
-// Buffer accesses of one cube (at most 2^31 bytes): a per-thread 32-bit byte
-// offset plus a wave-uniform one, instead of a 64-bit address per access.
-typedef int i2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float2 buffer_ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
-}
-__device__ __forceinline__ void buffer_st2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2_t, v), r, (int)voff, (int)soff, 0);
-}
 
 // Largest value over the work-group (slots: one float per wave, not reused
 // before the next barrier).
@@ -279,7 +267,8 @@ __device__ __forceinline__ void xy1_mfma_kernel(const float2* in, float2* out, c
       const float2* src = in + cube + (size_t)z * N * N + (16 * (row % 16) + y2_0 + row / 16) * N + j;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float2 t = src[16 * (4 * g + i)];
+        // one cube per launch: the signal is read non-temporally (see xy1_kernel)
+        const float2 t = ntotal <= K::A_TILES ? ld_stream(src + 16 * (4 * g + i)) : src[16 * (4 * g + i)];
         re[4 * b + i] = t.x;
         im[4 * b + i] = t.y;
       }
@@ -492,7 +481,9 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i)  // H at k2 = 4g+i, column beta = 16(4w+b) + j
-      cmul_soa(re[4 * b + i], im[4 * b + i], buffer_ld2(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2)));
+      cmul_soa(re[4 * b + i], im[4 * b + i],
+               ntotal <= M::TILES ? buffer_ld2_nt(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2))
+                                  : buffer_ld2(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2)));
   const int e2 = fp16_exp(block_max<Q::WAVES>(absmax16(re, im), red[1]));  // also: F read everywhere
   scale_soa(re, im, ldexpf(1.0f, e2));
   ops = dft16_ops<true>(optab);

@@ -169,11 +169,13 @@ struct Xy1Pass {
     int z, y2_0;
     tile_pos(tile, cube, z, y2_0);
     if constexpr (!INV) {
-      const float2* src = in + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
       if (nt) {
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(in + cube, (unsigned)(N * N * N * sizeof(float2)));
+        const unsigned voff = (unsigned)(((z * N + Y2 * ry1 + y2_0 + ry2l) * N + rt) * sizeof(float2));
 #pragma unroll
-        for (int m = 0; m < 16; ++m) r.v[m] = ld_stream(src + T * m);
+        for (int m = 0; m < 16; ++m) r.v[m] = buffer_ld2_nt(rs, voff, (unsigned)(T * m * sizeof(float2)));
       } else {
+        const float2* src = in + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
 #pragma unroll
         for (int m = 0; m < 16; ++m) r.v[m] = src[T * m];
       }
