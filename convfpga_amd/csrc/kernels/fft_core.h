@@ -63,6 +63,9 @@ __device__ __forceinline__ float2 buffer_ld2_nt(__amdgpu_buffer_rsrc_t r, unsign
 __device__ __forceinline__ void buffer_st2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2_t, v), r, (int)voff, (int)soff, 0);
 }
+__device__ __forceinline__ void buffer_st2_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2_t, v), r, (int)voff, (int)soff, 2);
+}
 
 // cos / sin of 2*pi*j/16, j = 0..7 (correctly rounded float literals)
 struct Tab16 {

@@ -156,7 +156,7 @@ struct Xy1Pass {
   float scale;
   int rt, row, ry1, ry2l;  // row identity: row = y2l*16 + y1, thread t holds x = t + T*m
   int ckx, cy2l;           // column identity: (x, y2l) with x%GB fastest, then y2l, then x/GB
-  bool nt;                 // one cube per launch: its signal is read non-temporally
+  bool nt;                 // one cube per launch: signal read and output written non-temporally
 
   __device__ __forceinline__ void tile_pos(int tile, size_t& cube, int& z, int& y2_0) const {
     cube = (size_t)(tile / K::A_TILES) * N * N * N;
@@ -217,9 +217,16 @@ struct Xy1Pass {
       for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1];
       __syncthreads();
       fft_line<N, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
-      float2* dst = out + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
+      if (nt) {  // the output is not read again by this convolution either
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(out + cube, (unsigned)(N * N * N * sizeof(float2)));
+        const unsigned voff = (unsigned)(((z * N + Y2 * ry1 + y2_0 + ry2l) * N + rt) * sizeof(float2));
 #pragma unroll
-      for (int m = 0; m < 16; ++m) dst[T * m] = cscale(r.v[m], scale);
+        for (int m = 0; m < 16; ++m) buffer_st2_nt(rs, voff, (unsigned)(T * m * sizeof(float2)), cscale(r.v[m], scale));
+      } else {
+        float2* dst = out + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dst[T * m] = cscale(r.v[m], scale);
+      }
     }
   }
 };
