@@ -28,6 +28,7 @@ the FP64 oracle.
 """
 import argparse
 import json
+import math
 import os
 import random
 import statistics
@@ -223,6 +224,13 @@ def run_single(a, torch, C, api):
     line["conv_roofline_frac_5pass_model"] = 11 * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS
     line["device_ms_per_step"] = sum(p["ms"] for p in passes)
     line["conv_per_s_without_pass_events"] = a.steps / wall_plain
+    # for information: the compulsory floor (read signal and spectrum, write the
+    # output: 3V at the HBM peak) and the arithmetic (two 3D FFTs, the product)
+    floor_us = 3 * 8.0 * P / (HBM_PEAK_GBS * 1e9) * 1e6
+    flops = 10.0 * P * math.log2(P) + 8.0 * P
+    line["compulsory_floor_3V"] = {"us": floor_us, "frac_of_measured": floor_us / (1e6 * wall / a.steps)}
+    line["flops_per_conv"] = flops
+    line["gflops"] = flops * a.steps / wall / 1e9
     line["copy_kernel_GBps"] = copy_gbs
     line["autotune_ms_per_conv"] = {api.PLAN_NAMES[p]: round(v, 5) for p, v in tune_ms.items()}
     line["passes"] = [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in passes]
