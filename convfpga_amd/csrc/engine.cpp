@@ -1595,10 +1595,11 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
     // segments: pass-major, instance-minor
     std::vector<CfBatchSeg> segs;
     int item = 0;
-    // tiles per work item: each item ends with a device-scope release (a write-back
-    // of the XCD's L2), so items are large; measured on MI355X (64 x 64^3, shared
-    // spectrum): 4 tiles 2.10 ms, 8 1.23, 16 0.83, 32 0.68, 64 0.86
-    int per_item = 32;
+    // tiles per work item: each item ends with one device-scope release (a
+    // write-back of the XCD's L2), so items are several tiles; measured on
+    // MI355X (64 x 64^3, shared spectrum / the mixed batch): 8 tiles 0.54 /
+    // 0.95 ms, 16 0.48 / 0.92, 32 0.52 / 1.12, 48 0.64 / 1.48
+    int per_item = 16;
     if (const char* e = getenv("CONVFPGA_BATCH_TILES_PER_ITEM")) per_item = std::max(1, atoi(e));
     for (int s = 0; s < kBatchKinds; ++s)
       for (unsigned i = 0; i < count; ++i) {

@@ -118,6 +118,11 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
             }
           }
         }
+        // one acquire for the work-group: it invalidates the caches this CU
+        // reads through (waited for, so it is complete before any wave loads);
+        // the other waves follow it through the barrier below
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
       }
       s_item = item;
       s_seg = seg;
@@ -127,7 +132,7 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
     const int item = __builtin_amdgcn_readfirstlane(s_item);
     const int seg = __builtin_amdgcn_readfirstlane(s_seg);
     if (item >= nitems) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave sees what the producer published
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // ordered after thread 0's acquire
     const CfBatchSeg sg = segs[seg];
     for (int i = threadIdx.x; i < sg.n; i += 256) tw_s[i] = ((const float2*)sg.tw)[i];
     __syncthreads();
@@ -149,8 +154,13 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
         default: break;
       }
     }
-    __threadfence();  // this thread's stores are visible device-wide ...
-    __syncthreads();  // ... for every thread of the item (thread 0 publishes next turn); LDS free again
+    // every wave waits until its stores have reached L2 (s_waitcnt 0: all
+    // counters), then thread 0's device-scope release at the top of the next
+    // turn writes the XCD's L2 back once for the whole item instead of once
+    // per wave
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();  // LDS free again
     prev = seg;
     prev_tiles = t1 - t0;
   }
