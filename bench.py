@@ -435,9 +435,11 @@ def run_batch(a, torch, C, api):
         return res
 
     # graph mode: spectra of each edge's autotuned plan; persistent mode:
-    # spectra in natural order (its layout), i.e. the axis plan for 128 and 256
+    # spectra in natural order (its layout), made under the axis plan for every
+    # edge (the autotuned plan of any edge from 64 up, four-step or z-blocked,
+    # may lay its spectrum out privately)
     inst_g = instances(make_specs())
-    natural = [n for n in by_n if n >= 128]  # edges whose autotuned plan may lay spectra out privately
+    natural = list(by_n)
     for n in natural:
         api.set_plan(n, api.CF_PLAN_AXIS5)
     inst = instances(make_specs())

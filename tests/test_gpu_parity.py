@@ -391,8 +391,10 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
     api = gpu.api
     sizes = [32, 64, 16, 128, 32, 256, 64, 32]
     print("case batch mode=%d plan256=%d sizes=%s" % (mode, plan256, sizes), flush=True)
-    # the persistent kernel reads spectra in natural order
-    plans = {256: plan256, 128: api.CF_PLAN_AXIS5} if mode == api.CF_BATCH_PERSISTENT else {256: plan256}
+    # the persistent kernel reads spectra in natural order (the autotuned plan
+    # of an edge from 64 up may lay them out privately)
+    plans = ({256: plan256, 128: api.CF_PLAN_AXIS5, 64: api.CF_PLAN_AXIS5} if mode == api.CF_BATCH_PERSISTENT
+             else {256: plan256})
     inst = _mixed_instances(gpu, oracle, sizes, plans)
     b = gpu.api.Batch([(ds, do, spec) for ds, do, spec, _, _ in inst], mode=mode)
     for rep in range(3):
@@ -425,7 +427,7 @@ def test_batch_spectrum_views_grouped(gpu, oracle, mode, plan128):
         host[n] = (s, f)
         dsig = _t(np.concatenate(s))
         dev[n] = (dsig, torch.zeros_like(dsig))
-        if mode == api.CF_BATCH_PERSISTENT and n >= 128:  # the persistent kernel reads natural-order spectra
+        if mode == api.CF_BATCH_PERSISTENT and n >= 64:  # the persistent kernel reads natural-order spectra
             api.set_plan(n, api.CF_PLAN_AXIS5)
         elif n == 128 and plan128:  # runs of a plan that goes through the workspace, batched
             api.set_plan(n, plan128)
@@ -550,7 +552,11 @@ def test_batch_shared_spectrum_in_place(gpu, oracle):
     n, k = 64, 4
     print("case batch shared spectrum in place", flush=True)
     f = oracle.uniform(n ** 3, 70001)
-    spec = gpu.Spectrum(n, _t(f))
+    gpu.api.set_plan(n, gpu.api.CF_PLAN_AXIS5)  # natural order, as the persistent batch reads it
+    try:
+        spec = gpu.Spectrum(n, _t(f))
+    finally:
+        gpu.api.set_plan(n, gpu.api.CF_PLAN_AUTO)
     sigs = [oracle.uniform(n ** 3, 70100 + i) for i in range(k)]
     ds = [_t(s) for s in sigs]
     b = gpu.api.Batch([(d, d, spec) for d in ds])
