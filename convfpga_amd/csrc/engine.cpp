@@ -67,7 +67,7 @@ struct SizeFns {
   hipFunction_t plane_fwd = nullptr, plane_inv = nullptr;                 // N in {32, 64, 128}
   hipFunction_t coly_fwd_zb = nullptr, coly_inv_zb = nullptr, colz_fwd_zb = nullptr, colz_conv_zb = nullptr;  // N >= 64
   hipFunction_t xy1_fwd[cfgeo::kSplitVariants] = {}, xy1_inv[cfgeo::kSplitVariants] = {};  // N = 256
-  hipFunction_t mid[cfgeo::kSplitVariants] = {}, hlay[cfgeo::kSplitVariants] = {};
+  hipFunction_t mid[cfgeo::kSplitVariants] = {}, mspec[cfgeo::kSplitVariants] = {};
   hipFunction_t mx_xy1_fwd = nullptr, mx_xy1_inv = nullptr, mx_mid = nullptr;  // N = 256, split-FP16 MFMA
   void* tw = nullptr;  // N float2 twiddles W_N^j
   int forced = CF_PLAN_AUTO;  // cf_set_plan
@@ -291,8 +291,8 @@ int size_res(unsigned n, SizeFns** out) {
     CFCHK(get_fn(&s.xy1_inv[vi], name));
     snprintf(name, sizeof name, "cf_mid_%u_%s", n, sfx);
     CFCHK(get_fn(&s.mid[vi], name));
-    snprintf(name, sizeof name, "cf_hlayout_%u_%s", n, sfx);
-    CFCHK(get_fn(&s.hlay[vi], name));
+    snprintf(name, sizeof name, "cf_midspec_%u_%s", n, sfx);
+    CFCHK(get_fn(&s.mspec[vi], name));
   }
   if (n == 256) {
     CFCHK(get_fn(&s.mx_xy1_fwd, "cf_xy1_fwd_256_s24"));
@@ -430,12 +430,16 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
   return split;
 }
 
-// Filter setup of `plan`: natural-order forward FFT3 (in place in dst), plus the
-// re-layout into the middle pass's order for the four-step plans (via work).
+// Filter setup of `plan`: the forward FFT3 of `count` filters into dst, in the
+// layout the plan's product pass reads (in place when src == dst).  Natural
+// order by the axis passes; z-blocked for AXIS5Z; for the four-step plans the
+// first pass of the convolution into the workspace, then the forward half of
+// the middle pass straight into the middle pass's spectrum order (4 volume
+// transfers instead of 6 for the axis passes and 2 for a re-layout).
 void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, const void* src, void* dst, void* work,
                      std::vector<Pass>& v) {
+  const double io = 2.0 * cube_bytes(n) * count;
   if (plan == CF_PLAN_AXIS5Z) {  // x into the workspace, y into dst (z-blocked), z in place
-    const double io = 2.0 * cube_bytes(n) * count;
     const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
     const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
     v.push_back({f.row_fwd, rT, rt, src, work, nullptr, 0, 1.0f, "x_fwd", io});
@@ -447,9 +451,11 @@ void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, con
     fwd_passes(f, n, count, src, dst, v);
     return;
   }
-  fwd_passes(f, n, count, src, work, v);
-  v.push_back({f.hlay[split_variant(plan)], (unsigned)(cube_elems(n) / 256), 256, work, dst, nullptr, 0, 1.0f,
-               "h_layout", 2.0 * cube_bytes(n) * count, false});
+  const int vi = split_variant(plan);
+  const unsigned at = cfgeo::split_a_threads((int)n, vi), aT = cfgeo::split_a_tiles((int)n, vi);
+  const unsigned bt = cfgeo::split_b_threads((int)n, vi), bT = cfgeo::split_b_tiles((int)n, vi);
+  v.push_back({f.xy1_fwd[vi], aT, at, src, work, nullptr, 0, 1.0f, "xy1_fwd", io});
+  v.push_back({f.mspec[vi], bT, bt, work, dst, nullptr, 0, 1.0f, "y2z_fwd", io});
 }
 
 int run_passes(const std::vector<Pass>& v, unsigned n, unsigned nb, hipStream_t s, const void* tw,

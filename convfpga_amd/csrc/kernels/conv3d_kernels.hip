@@ -108,8 +108,8 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_mid_##N##_##SUFFIX(CF_ARGS) { \
     cfk::mid_kernel<N, GA, GB, GM, (GM <= 2)>(in, out, H, tw, hstride, ntotal);                                 \
   }                                                                                                             \
-  extern "C" __global__ __launch_bounds__(256) void cf_hlayout_##N##_##SUFFIX(CF_ARGS) {                        \
-    cfk::h_layout_body<N, GM>(in, out);                                                                         \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_midspec_##N##_##SUFFIX(CF_ARGS) { \
+    cfk::mid_spec_kernel<N, GA, GB, GM>(in, out, tw, ntotal);                                                  \
   }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points

@@ -15,6 +15,8 @@
 //             Y2-point DFT over k2, inverse z FFT -- one contiguous tile
 //   xy1_inv : twiddle W_N^(-y2 k1), inverse 16-point DFT over k1, inverse x
 //             FFT, final scale, natural order out
+//   filter setup: xy1_fwd, then mid_spec (the forward half of mid_conv), which
+//             stores the spectrum in the order mid_conv reads it
 // The intermediate between the passes (in the workspace) is indexed by
 // (x, k1, y2, z) and laid out [k1][x/GB][z][y2][x%GB]: xy1_* move runs of
 // GA*GB contiguous points.  A mid_conv tile is (k1, GM adjacent x) with GM
@@ -386,6 +388,29 @@ struct MidPass {
 #pragma unroll
     for (int m = 0; m < 16; ++m) dst[T * Y2 * GB * m] = r.v[m];
   }
+  // Filter setup: the forward half only (z FFT, Y2-point DFT over y2), whose
+  // result is the tile's block of the full 3D spectrum; it is stored where
+  // process() reads it, [k1][x/GM][k2][kz][x%GM] with N^3 points per filter.
+  __device__ __forceinline__ void spectrum(int tile, Regs& r) const {
+    constexpr int PITCH = M::PITCH;
+    const PitchLds<PITCH> view{lds + l};
+    fft_line<N, false>(r.v, zt, view, tw_s);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) lds[(zt + T * m) * PITCH + l] = r.v[m];
+    __syncthreads();
+    float2* spec = out + (size_t)(tile / M::TILES) * N * N * N + (size_t)(tile % M::TILES) * M::HTILE;
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      const int row = q + M::THREADS * rr;
+      const float2* rowp = lds + (row / GM) * PITCH + row % GM;
+      float2 u[Y2];
+#pragma unroll
+      for (int y2 = 0; y2 < Y2; ++y2) u[y2] = rowp[y2 * GM];
+      DFT<Y2, false>::run(u);  // u[k2]: spectrum at (x, k1 + 16 k2, kz)
+#pragma unroll
+      for (int k2 = 0; k2 < Y2; ++k2) spec[row + k2 * N * GM] = u[k2];
+    }
+  }
 };
 
 template <int N, int GA, int GB, int GM, bool HPF = false>
@@ -407,19 +432,23 @@ __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const 
   k.process(tile, a);
 }
 
-// Spectrum re-layout for the middle pass: natural FFT3 order -> [k1][x/GM][k2][kz][x%GM].
-template <int N, int GM>
-__device__ __forceinline__ void h_layout_body(const float2* __restrict__ in, float2* __restrict__ out) {
-  constexpr int NXM = N / GM, Y2 = N / 16;
-  const int i = blockIdx.x * 256 + threadIdx.x;  // output index within the cube
-  const size_t cube = (size_t)blockIdx.y * N * N * N;
-  const int xl = i % GM;
-  const int kz = (i / GM) % N;
-  const int k2 = (i / (GM * N)) % Y2;
-  const int xh = (i / (GM * N * Y2)) % NXM;
-  const int k1 = i / (GM * N * Y2 * NXM);
-  const int x = xh * GM + xl, ky = k1 + 16 * k2;
-  out[cube + i] = in[cube + ((size_t)kz * N + ky) * N + x];
+// Filter setup of a four-step plan, second pass: after xy1_kernel<INV=false>
+// has written the filter's intermediate, every middle-pass tile transforms its
+// block forward and stores it in the spectrum order the middle pass reads.
+template <int N, int GA, int GB, int GM>
+__device__ __forceinline__ void mid_spec_kernel(const float2* in, float2* out, const float2* tw, int ntotal) {
+  using M = MidGeoN<N, GB, GM>;
+  static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
+  __shared__ float2 lds[M::LDS];
+  __shared__ float2 tw_s[N];
+  stage_twiddles<N, M::THREADS>(tw_s, tw);
+  const int x = threadIdx.x;
+  const MidPass<N, GA, GB, GM, false> k{in, out, nullptr, tw_s, lds, 0, x % M::LINES, x / M::LINES, x, false};
+  const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
+  if (tile >= ntotal) return;
+  typename MidPass<N, GA, GB, GM, false>::Regs a;
+  k.load(tile, a);
+  k.spectrum(tile, a);
 }
 
 }  // namespace cfk

@@ -329,6 +329,44 @@ def test_dropin_256_on_each_plan(gpu, oracle, plan):
         gpu.api.set_plan(n, gpu.api.CF_PLAN_AUTO)
 
 
+def _spectrum_copy(spec, n, count):
+    """Device copy of a spectrum handle's own buffer (its plan's layout)."""
+    import ctypes
+    import torch
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = torch.empty(count * n ** 3, dtype=torch.complex64, device="cuda")
+    torch.cuda.synchronize()
+    assert hip.hipMemcpy(out.data_ptr(), spec.data_ptr(), out.numel() * 8, 3) == 0  # 3: device to device
+    return _np(out)
+
+
+@pytest.mark.parametrize("n,plan", [(128, 9), (256, 4), (256, 7), (256, 8)])  # g28m4, g24, g24m2, g24m1
+def test_four_step_spectrum_layout(gpu, oracle, n, plan):
+    """The filter spectrum of a four-step plan (the convolution's first pass,
+    then the forward half of its middle pass) equals the FP64 FFT3 of each
+    filter permuted into the middle pass's order [k1][x/GM][k2][kz][x%GM],
+    ky = k1 + 16 k2, for both filters of a two-filter handle."""
+    api = gpu.api
+    name = api.PLAN_NAMES[plan]  # split3_g<GA><GB>[m<GM>]
+    gm = int(name.split("m")[-1]) if "m" in name.split("_g")[1] else int(name[-1])
+    print("case four-step spectrum N=%d plan=%s GM=%d" % (n, name, gm), flush=True)
+    P, y2 = n ** 3, n // 16
+    filts = [oracle.uniform(P, 83000 + 7 * n + k) for k in range(2)]
+    api.set_plan(n, plan)
+    try:
+        spec = gpu.Spectrum(n, _t(np.concatenate(filts)), count=2)
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    assert spec.plan == plan
+    got = _spectrum_copy(spec, n, 2)
+    spec.close()
+    for k, f in enumerate(filts):
+        ref = oracle.fft3d_f64(f, -1).reshape(n, y2, 16, n // gm, gm)  # (kz, k2, k1, x/GM, x%GM)
+        want = np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
+        assert oracle.rel_l2(got[k * P:(k + 1) * P], want) <= REL_L2_FP32, k
+
+
 @pytest.mark.parametrize("plan", [2, 5])
 def test_svm_batch_on_3pass_plans(gpu, oracle, plan):
     n = 64 if plan == 2 else 256
