@@ -432,7 +432,8 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
 
 // Filter setup of `plan`: the forward FFT3 of `count` filters into dst, in the
 // layout the plan's product pass reads (in place when src == dst).  Natural
-// order by the axis passes; z-blocked for AXIS5Z; for the four-step plans the
+// order by the axis passes (PLANE3: the plane pass and z); z-blocked for
+// AXIS5Z; for the four-step plans the
 // first pass of the convolution into the workspace, then the forward half of
 // the middle pass straight into the middle pass's spectrum order (4 volume
 // transfers instead of 6 for the axis passes and 2 for a re-layout).
@@ -445,6 +446,13 @@ void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, con
     v.push_back({f.row_fwd, rT, rt, src, work, nullptr, 0, 1.0f, "x_fwd", io});
     v.push_back({f.coly_fwd_zb, cT, ct, work, dst, nullptr, 0, 1.0f, "y_fwd", io});
     v.push_back({f.colz_fwd_zb, cT, ct, dst, dst, nullptr, 0, 1.0f, "z_fwd", io});
+    return;
+  }
+  if (plan == CF_PLAN_PLANE3) {  // x and y in one pass per plane, then z
+    const unsigned pt = cfgeo::plane_threads(n), pT = cfgeo::plane_tiles(n);
+    const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
+    v.push_back({f.plane_fwd, pT, pt, src, dst, nullptr, 0, 1.0f, "xy_fwd", io});
+    v.push_back({f.colz_fwd, cT, ct, dst, dst, nullptr, 0, 1.0f, "z_fwd", io});
     return;
   }
   if (!is_split(plan)) {
