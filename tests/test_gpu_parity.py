@@ -341,17 +341,32 @@ def _spectrum_copy(spec, n, count):
     return _np(out)
 
 
-@pytest.mark.parametrize("n,plan", [(128, 9), (256, 4), (256, 7), (256, 8)])  # g28m4, g24, g24m2, g24m1
-def test_four_step_spectrum_layout(gpu, oracle, n, plan):
-    """The filter spectrum of a four-step plan (the convolution's first pass,
-    then the forward half of its middle pass) equals the FP64 FFT3 of each
-    filter permuted into the middle pass's order [k1][x/GM][k2][kz][x%GM],
-    ky = k1 + 16 k2, for both filters of a two-filter handle."""
+def _spectrum_order(api, n, plan, spec64):
+    """The FP64 spectrum (natural order, index (kz*N + ky)*N + kx) permuted into
+    the layout a spectrum of `plan` keeps: natural for AXIS5 / PLANE3,
+    ((kx/16)*N + ky)*16N + kz*16 + kx%16 for AXIS5Z, [k1][kx/GM][k2][kz][kx%GM]
+    (ky = k1 + 16 k2) for the four-step plans."""
+    name = api.PLAN_NAMES[plan]
+    if name in ("axis5", "plane3"):
+        return spec64
+    if name == "axis5z":
+        return np.ascontiguousarray(spec64.reshape(n, n, n // 16, 16).transpose(2, 1, 0, 3)).reshape(-1)
+    tail = name.split("_g")[1]  # <GA><GB>[m<GM>]
+    gm = int(tail.split("m")[1]) if "m" in tail else int(tail[1])
+    ref = spec64.reshape(n, n // 16, 16, n // gm, gm)  # (kz, k2, k1, kx/GM, kx%GM)
+    return np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
+
+
+@pytest.mark.parametrize("n,plan", [(64, 1), (64, 2), (64, 12), (128, 9), (256, 4), (256, 7), (256, 8)])
+def test_spectrum_layout_of_each_plan(gpu, oracle, n, plan):
+    """The filter spectrum a plan keeps (axis / plane passes: natural order;
+    AXIS5Z: z-blocked; four-step: the convolution's first pass, then the
+    forward half of its middle pass) equals the FP64 FFT3 of each filter
+    permuted into that plan's layout, for both filters of a two-filter
+    handle."""
     api = gpu.api
-    name = api.PLAN_NAMES[plan]  # split3_g<GA><GB>[m<GM>]
-    gm = int(name.split("m")[-1]) if "m" in name.split("_g")[1] else int(name[-1])
-    print("case four-step spectrum N=%d plan=%s GM=%d" % (n, name, gm), flush=True)
-    P, y2 = n ** 3, n // 16
+    print("case spectrum layout N=%d plan=%s" % (n, api.PLAN_NAMES[plan]), flush=True)
+    P = n ** 3
     filts = [oracle.uniform(P, 83000 + 7 * n + k) for k in range(2)]
     api.set_plan(n, plan)
     try:
@@ -362,8 +377,7 @@ def test_four_step_spectrum_layout(gpu, oracle, n, plan):
     got = _spectrum_copy(spec, n, 2)
     spec.close()
     for k, f in enumerate(filts):
-        ref = oracle.fft3d_f64(f, -1).reshape(n, y2, 16, n // gm, gm)  # (kz, k2, k1, x/GM, x%GM)
-        want = np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
+        want = _spectrum_order(api, n, plan, oracle.fft3d_f64(f, -1))
         assert oracle.rel_l2(got[k * P:(k + 1) * P], want) <= REL_L2_FP32, k
 
 
