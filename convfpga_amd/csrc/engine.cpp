@@ -150,28 +150,28 @@ std::string lower(const std::string& s) {
 void pipe_release();
 
 void release_all() {
-  if (g.ready || g.stream) hipSetDevice(g.device);
-  if (g.stream) hipStreamSynchronize(g.stream);
+  if (g.ready || g.stream) (void)hipSetDevice(g.device);
+  if (g.stream) (void)hipStreamSynchronize(g.stream);
   pipe_release();
   for (auto& kv : g.sizes)
-    if (kv.second.tw) hipFree(kv.second.tw);
+    if (kv.second.tw) (void)hipFree(kv.second.tw);
   g.sizes.clear();
-  if (g.d_data) hipFree(g.d_data);
-  if (g.d_filt) hipFree(g.d_filt);
-  if (g.d_err) hipFree(g.d_err);
-  if (g.d_work) hipFree(g.d_work);
+  if (g.d_data) (void)hipFree(g.d_data);
+  if (g.d_filt) (void)hipFree(g.d_filt);
+  if (g.d_err) (void)hipFree(g.d_err);
+  if (g.d_work) (void)hipFree(g.d_work);
   g.d_work = nullptr;
   g.d_work_bytes = 0;
   g.work_used = false;
   g.work_stream = nullptr;
-  if (g.work_ev) hipEventDestroy(g.work_ev);
+  if (g.work_ev) (void)hipEventDestroy(g.work_ev);
   g.work_ev = nullptr;
-  if (g.h_sig) hipHostFree(g.h_sig);
-  if (g.h_filt) hipHostFree(g.h_filt);
-  if (g.h_out) hipHostFree(g.h_out);
+  if (g.h_sig) (void)hipHostFree(g.h_sig);
+  if (g.h_filt) (void)hipHostFree(g.h_filt);
+  if (g.h_out) (void)hipHostFree(g.h_out);
   for (int b = 0; b < 2; ++b) {
-    if (g.stage[b]) hipHostFree(g.stage[b]);
-    if (g.stage_ev[b]) hipEventDestroy(g.stage_ev[b]);
+    if (g.stage[b]) (void)hipHostFree(g.stage[b]);
+    if (g.stage_ev[b]) (void)hipEventDestroy(g.stage_ev[b]);
     g.stage[b] = nullptr;
     g.stage_ev[b] = nullptr;
     g.stage_used[b] = false;
@@ -180,18 +180,18 @@ void release_all() {
   g.d_data_bytes = g.d_filt_bytes = g.d_err_bytes = g.h_sig_bytes = g.h_filt_bytes = g.h_out_bytes = 0;
   for (auto& e : g.ev)
     if (e) {
-      hipEventDestroy(e);
+      (void)hipEventDestroy(e);
       e = nullptr;
     }
-  for (auto e : g.prof_ev) hipEventDestroy(e);
+  for (auto e : g.prof_ev) (void)hipEventDestroy(e);
   g.prof_ev.clear();
   g.prof_max = g.prof_calls = 0;
-  if (g.stream) hipStreamDestroy(g.stream);
+  if (g.stream) (void)hipStreamDestroy(g.stream);
   g.stream = nullptr;
-  if (g.mod) hipModuleUnload(g.mod);
+  if (g.mod) (void)hipModuleUnload(g.mod);
   g.mod = nullptr;
   g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_reset = nullptr;
-  if (g.capture) hipStreamDestroy(g.capture);
+  if (g.capture) (void)hipStreamDestroy(g.capture);
   g.capture = nullptr;
   g.ready = false;
   g.svm = false;
@@ -478,7 +478,7 @@ int run_passes(const std::vector<Pass>& v, unsigned n, unsigned nb, hipStream_t 
 
 int ensure_device(void** p, size_t* have, size_t need) {
   if (*have >= need) return CF_OK;
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
   *have = 0;
   hipError_t e = hipMalloc(p, need);
@@ -489,7 +489,7 @@ int ensure_device(void** p, size_t* have, size_t need) {
 
 int ensure_pinned(void** p, size_t* have, size_t need) {
   if (*have >= need) return CF_OK;
-  if (*p) hipHostFree(*p);
+  if (*p) (void)hipHostFree(*p);
   *p = nullptr;
   *have = 0;
   hipError_t e = hipHostMalloc(p, need, hipHostMallocMapped);
@@ -597,7 +597,7 @@ int acquire_work(size_t bytes, hipStream_t s, void** out) {
   if (g.d_work_bytes < bytes) {
     if (g.d_work) {
       HIPCHK(hipDeviceSynchronize());  // earlier users on any stream are done
-      hipFree(g.d_work);
+      (void)hipFree(g.d_work);
     }
     g.d_work = nullptr;
     g.d_work_bytes = 0;
@@ -666,10 +666,10 @@ int autotune(unsigned n, SizeFns* f) {
   hipStream_t s = g.stream;
   int rc = CF_OK;
   auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    if (sig) hipFree(sig);
-    if (out) hipFree(out);
-    if (H) hipFree(H);
+    (void)hipStreamSynchronize(s);
+    if (sig) (void)hipFree(sig);
+    if (out) (void)hipFree(out);
+    if (H) (void)hipFree(H);
   };
   if (hipMalloc(&sig, V) != hipSuccess || hipMalloc(&out, V) != hipSuccess || hipMalloc(&H, V) != hipSuccess) {
     cleanup();
@@ -760,22 +760,22 @@ struct PipeState {
 PipeState gp;
 
 void pipe_release() {
-  if (gp.s_in) hipStreamSynchronize(gp.s_in);
-  if (gp.s_out) hipStreamSynchronize(gp.s_out);
+  if (gp.s_in) (void)hipStreamSynchronize(gp.s_in);
+  if (gp.s_out) (void)hipStreamSynchronize(gp.s_out);
   for (int b = 0; b < 2; ++b) {
-    if (gp.stage_in[b]) hipHostFree(gp.stage_in[b]);
-    if (gp.stage_out[b]) hipHostFree(gp.stage_out[b]);
-    if (gp.ev_in[b]) hipEventDestroy(gp.ev_in[b]);
-    if (gp.ev_out[b]) hipEventDestroy(gp.ev_out[b]);
+    if (gp.stage_in[b]) (void)hipHostFree(gp.stage_in[b]);
+    if (gp.stage_out[b]) (void)hipHostFree(gp.stage_out[b]);
+    if (gp.ev_in[b]) (void)hipEventDestroy(gp.ev_in[b]);
+    if (gp.ev_out[b]) (void)hipEventDestroy(gp.ev_out[b]);
   }
   for (int i = 0; i < kPipeSlots; ++i) {
-    if (gp.ev_loaded[i]) hipEventDestroy(gp.ev_loaded[i]);
-    if (gp.ev_done[i]) hipEventDestroy(gp.ev_done[i]);
-    if (gp.ev_freed[i]) hipEventDestroy(gp.ev_freed[i]);
+    if (gp.ev_loaded[i]) (void)hipEventDestroy(gp.ev_loaded[i]);
+    if (gp.ev_done[i]) (void)hipEventDestroy(gp.ev_done[i]);
+    if (gp.ev_freed[i]) (void)hipEventDestroy(gp.ev_freed[i]);
   }
-  if (gp.slots) hipFree(gp.slots);
-  if (gp.s_in) hipStreamDestroy(gp.s_in);
-  if (gp.s_out) hipStreamDestroy(gp.s_out);
+  if (gp.slots) (void)hipFree(gp.slots);
+  if (gp.s_in) (void)hipStreamDestroy(gp.s_in);
+  if (gp.s_out) (void)hipStreamDestroy(gp.s_out);
   gp = PipeState();
 }
 
@@ -797,7 +797,7 @@ int pipe_ready(size_t slot_bytes) {
   if (gp.slots_bytes < need) {
     if (gp.slots) {
       HIPCHK(hipDeviceSynchronize());
-      hipFree(gp.slots);
+      (void)hipFree(gp.slots);
     }
     gp.slots = nullptr;
     gp.slots_bytes = 0;
@@ -973,7 +973,7 @@ int pipe_run(PipeJob j, double* pass_ms, double* wall_ms) {
     std::vector<hipEvent_t>& v;
     ~Events() {
       for (auto e : v)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     }
   } hold{tev};
   for (auto& e : tev) HIPCHK(hipEventCreate(&e));
@@ -1224,7 +1224,7 @@ extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filter
   }
   if (rc == CF_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "spectrum setup failed");
   if (rc != CF_OK) {
-    hipFree(h->d);
+    (void)hipFree(h->d);
     delete h;
     return rc;
   }
@@ -1236,8 +1236,8 @@ extern "C" void cf_spectrum_destroy(cf_spectrum_t h) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!h) return;
   if (h->d && !h->view) {
-    hipSetDevice(g.device);
-    hipFree(h->d);
+    (void)hipSetDevice(g.device);
+    (void)hipFree(h->d);
   }
   delete h;
 }
@@ -1453,7 +1453,7 @@ extern "C" int cf_profile_begin(unsigned max_calls) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (max_calls == 0 || max_calls > 100000) return fail(CF_ERR_ARG, "max_calls must be in 1..100000");
   CFCHK(require_ready());
-  for (auto e : g.prof_ev) hipEventDestroy(e);
+  for (auto e : g.prof_ev) (void)hipEventDestroy(e);
   g.prof_ev.assign((size_t)max_calls * (CF_MAX_PASSES + 1), nullptr);
   // timing-only events: no system-scope fence (cache writeback) when they
   // complete, which would otherwise stall the next pass for microseconds
@@ -1483,7 +1483,7 @@ extern "C" int cf_profile_end(cf_pass_times_t* avg, unsigned* ncalls) {
       avg->bytes[i] = g.prof_bytes[i];
     }
   }
-  for (auto e : g.prof_ev) hipEventDestroy(e);
+  for (auto e : g.prof_ev) (void)hipEventDestroy(e);
   g.prof_ev.clear();
   g.prof_max = g.prof_calls = 0;
   return rc;
@@ -1565,11 +1565,11 @@ struct cf_batch_s {
 
 static void batch_free(cf_batch_s* b) {
   if (!b) return;
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  if (b->d_segs) hipFree(b->d_segs);
-  if (b->d_ctr) hipFree(b->d_ctr);
-  if (b->d_work) hipFree(b->d_work);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_segs) (void)hipFree(b->d_segs);
+  if (b->d_ctr) (void)hipFree(b->d_ctr);
+  if (b->d_work) (void)hipFree(b->d_work);
   delete b;
 }
 
@@ -1577,7 +1577,7 @@ static int batch_capture_end(cf_batch_s* b, int rc) {
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(g.capture, &graph);
   if (rc != CF_OK) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (e != hipSuccess) return fail(CF_ERR_RUNTIME, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -1758,8 +1758,8 @@ extern "C" int cf_batch_check(cf_batch_t b) {
 extern "C" void cf_batch_destroy(cf_batch_t b) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!b) return;
-  hipSetDevice(g.device);
-  if (b->launched) hipStreamSynchronize(b->last);
+  (void)hipSetDevice(g.device);
+  if (b->launched) (void)hipStreamSynchronize(b->last);
   batch_free(b);
 }
 #pragma GCC visibility pop
