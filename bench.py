@@ -69,8 +69,8 @@ def parse():
 def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps, split_out_host=None):
     """Times the oracle's FP32 CPU convolution (port of src/fft_conv3D.cpp:50-177)
     on `reps` convolutions of one cube, and checks the GPU output on the same
-    inputs.  With split_check_leg (the c5 checker), the only places bench.py
-    uses the oracle."""
+    inputs.  With batch_cpu_leg and split_check_leg (the batch and c5
+    checkers), the only places bench.py uses the oracle."""
     import numpy as np
     from oracle import oracle as O
     O.build()
@@ -109,6 +109,29 @@ def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps, split_out_host=
         if split_out_host is not None:
             check["rel_l2_vs_fp64_split_fp16_mfma"] = O.rel_l2(np.ascontiguousarray(split_out_host[:P]), ref64)
     return res, check
+
+
+def batch_cpu_leg(by_n, shared, seed, outs, reps):
+    """cpu_baseline of a batch workload: the oracle's FP32 convolution of one
+    cube of the batch's largest edge, timed as cpu_baseline_leg times it, and
+    the checker: every instance's output in every mode against the FP64
+    oracle, on the inputs the bench generated on the device (the oracle's
+    generator is the device generator's bit-exact port)."""
+    import numpy as np
+    from oracle import oracle as O
+    nmax = max(by_n)
+    res, _ = cpu_baseline_leg(nmax, None, seed + nmax, seed + 7 * nmax, reps)
+    worst = {mode: 0.0 for mode in outs}
+    for n, cnt in by_n.items():
+        P = n ** 3
+        sigs = O.uniform(cnt * P, seed + n)  # the stream the bench filled the edge's signals from
+        filts = O.uniform((1 if shared else cnt) * P, seed + 7 * n)
+        for k in range(cnt):
+            f = filts[:P] if shared else filts[k * P:(k + 1) * P]
+            ref = O.conv3d_f64(n, np.ascontiguousarray(sigs[k * P:(k + 1) * P]), np.ascontiguousarray(f), 1.0 / P)
+            for mode, o in outs.items():
+                worst[mode] = max(worst[mode], O.rel_l2(np.ascontiguousarray(o[n][k * P:(k + 1) * P]), ref))
+    return res, {"rel_l2_vs_fp64_max_by_mode": worst, "instances_checked": sum(by_n.values())}
 
 
 def cpu_model():
@@ -462,14 +485,24 @@ def run_batch(a, torch, C, api):
         sync_check()
         return (t1 - t0) / a.steps
 
+    outs = {}  # each mode's outputs (cleared before it runs), checked in the cpu_baseline leg
+
+    def run_mode(name, launch, sync_check):
+        for n in by_n:
+            out[n].zero_()
+        results[name] = timed(launch, sync_check)
+        if not a.no_cpu_baseline:
+            torch.cuda.synchronize()
+            outs[name] = {n: out[n].cpu().numpy() for n in by_n}
+
     bp = api.Batch([(s, o, h) for s, o, h in inst], mode=api.CF_BATCH_PERSISTENT)
-    results["persistent_one_launch"] = timed(bp.launch, bp.check)
+    run_mode("persistent_one_launch", bp.launch, bp.check)
     bg = api.Batch([(s, o, h) for s, o, h in inst_sorted], mode=api.CF_BATCH_GRAPH)
-    results["graph"] = timed(bg.launch, bg.check)
+    run_mode("graph", bg.launch, bg.check)
     if shared:  # the batch as one sequence of batched passes of the autotuned plan
         sp = C.Spectrum(64, filt[64][:64 ** 3])
-        results["batched_passes_" + api.PLAN_NAMES[sp.plan]] = timed(
-            lambda: C.conv3d_device(sp, sig[64], out[64], how_many=len(sizes)), torch.cuda.synchronize)
+        run_mode("batched_passes_" + api.PLAN_NAMES[sp.plan],
+                 lambda: C.conv3d_device(sp, sig[64], out[64], how_many=len(sizes)), torch.cuda.synchronize)
     per_size = {}
     if not shared:  # per-edge throughput: each edge group alone, in each mode
         for mode, mname, group in ((api.CF_BATCH_GRAPH, "graph", inst_g),
@@ -510,7 +543,7 @@ def run_batch(a, torch, C, api):
         line["per_size"] = per_size
     bp.close()
     bg.close()
-    return line, None
+    return line, ("batch", by_n, shared, a.seed, outs)
 
 
 def run_split(a, torch, C, api):
@@ -647,9 +680,12 @@ def main():
     else:
         line, cpu = (run_single if a.workload in SINGLE else run_batch)(a, torch, C, api)
         line["cpu_baseline"] = None
-        if cpu is not None and not a.no_cpu_baseline:
-            base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps,
-                                           cpu[4] if len(cpu) > 4 else None)
+        if not a.no_cpu_baseline:
+            if cpu[0] == "batch":
+                base, check = batch_cpu_leg(cpu[1], cpu[2], cpu[3], cpu[4], a.cpu_reps)
+            else:
+                base, check = cpu_baseline_leg(cpu[0], cpu[1], cpu[2], cpu[3], a.cpu_reps,
+                                               cpu[4] if len(cpu) > 4 else None)
             line["cpu_baseline"] = base
             line.update(check)
     C.fpga_final()

@@ -37,3 +37,16 @@ def test_single_cube_line():
     assert cb["kind"] == "port" and cb["value"] > 0 and cb["cores"] >= 1
     assert math.isfinite(d["rel_l2_vs_fp64"]) and d["rel_l2_vs_fp64"] <= 1e-5
     assert d["snr_db_vs_fp32_cpu"] > 120
+
+
+@pytest.mark.gpu
+def test_batch_line_checks_every_instance():
+    print("case bench c3_64x64", flush=True)
+    r = _bench("--workload", "c3_64x64", "--steps", "3", "--warmup", "1", "--cpu-reps", "2")
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["value"] > 0 and d["roofline"]["bound"] == "hbm" and d["cpu_baseline"]["value"] > 0
+    assert d["instances_checked"] == 64
+    worst = d["rel_l2_vs_fp64_max_by_mode"]
+    assert set(worst) == set(d["modes_ms_per_batch"])
+    assert all(math.isfinite(v) and v <= 1e-5 for v in worst.values()), worst
