@@ -217,6 +217,7 @@ class Spectrum:
 
     def __init__(self, N, filters, count=1, on_device=True, stream=None):
         self._parent = None
+        self._nviews, self._close_pending = 0, False
         h = ctypes.c_void_p()
         if on_device:
             ptr = _dev(filters)
@@ -231,13 +232,18 @@ class Spectrum:
     def _wrap(cls, h, N, parent):
         obj = cls.__new__(cls)
         obj._h, obj.N, obj.count, obj._parent = h, N, 1, parent
+        obj._nviews, obj._close_pending = 0, False
         return obj
 
     def view(self, index):
         """Spectrum handle for filter `index` alone, sharing this one's memory
-        (cf_spectrum_view); it keeps this object alive."""
+        (cf_spectrum_view); the C handle of this one is kept until its last
+        view is closed."""
+        if not self._h or self._close_pending:
+            raise ConvError("cf_spectrum_view: the spectrum is closed")
         h = ctypes.c_void_p()
         _check(lib().cf_spectrum_view(self._h, index, ctypes.byref(h)), "cf_spectrum_view")
+        self._nviews += 1
         return Spectrum._wrap(h, self.N, self)
 
     @property
@@ -252,9 +258,21 @@ class Spectrum:
         return lib().cf_spectrum_plan(self._h)
 
     def close(self):
-        if self._h:
-            lib().cf_spectrum_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        """Destroys the handle.  A view is valid only while its parent lives
+        (cf_spectrum_view), so a spectrum with open views is destroyed when
+        the last of them is closed."""
+        if not self._h:
+            return
+        if self._nviews:
+            self._close_pending = True
+            return
+        lib().cf_spectrum_destroy(self._h)
+        self._h = ctypes.c_void_p()
+        parent, self._parent = self._parent, None
+        if parent is not None:
+            parent._nviews -= 1
+            if parent._close_pending and parent._nviews == 0:
+                parent.close()
 
     def __del__(self):
         try:

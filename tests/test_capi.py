@@ -146,3 +146,32 @@ def test_batch_descriptor_is_64_bytes():
     # CfBatchSeg (convfpga_amd/csrc/batch_desc.h) is shared by host and device code
     src = open(os.path.join(ROOT, "convfpga_amd", "csrc", "batch_desc.h")).read()
     assert "sizeof(CfBatchSeg) == 64" in src
+
+
+def test_spectrum_close_waits_for_its_views(monkeypatch):
+    """A view is valid only while its parent lives (cf_spectrum_view): the
+    Python mirror destroys a parent closed with open views when the last view
+    is closed, and refuses new views of it meanwhile."""
+    destroyed = []
+
+    class FakeLib:
+        def cf_spectrum_view(self, parent, index, out):
+            out._obj.value = 1000 + index
+            return 0
+
+        def cf_spectrum_destroy(self, h):
+            destroyed.append(h.value)
+
+    monkeypatch.setattr(api, "lib", lambda: FakeLib())
+    parent = api.Spectrum._wrap(ctypes.c_void_p(7), 8, None)
+    v0, v1 = parent.view(0), parent.view(1)
+    parent.close()
+    assert destroyed == [] and parent.handle
+    with pytest.raises(api.ConvError):
+        parent.view(0)
+    v0.close()
+    assert destroyed == [1000]
+    v1.close()
+    assert destroyed == [1000, 1001, 7] and not parent.handle
+    parent.close()  # idempotent
+    assert destroyed == [1000, 1001, 7]

@@ -461,6 +461,27 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
     b.close()
 
 
+def test_spectrum_views_outlive_a_closed_parent(gpu, oracle):
+    """Closing a two-filter spectrum while views of it are open keeps its
+    memory until the last view is closed: a view still convolves correctly."""
+    n = 32
+    P = n ** 3
+    print("case spectrum views after parent close", flush=True)
+    f = [oracle.uniform(P, 84000 + k) for k in range(2)]
+    s = oracle.uniform(P, 84100)
+    parent = gpu.Spectrum(n, _t(np.concatenate(f)), count=2)
+    v0, v1 = parent.view(0), parent.view(1)
+    parent.close()
+    assert parent.handle
+    d = _t(s)
+    o = d.clone()
+    gpu.conv3d_device(v1, d, o)
+    assert oracle.rel_l2(_np(o), oracle.conv3d_f64(n, s, f[1], 1.0 / P)) <= REL_L2_FP32
+    v0.close()
+    v1.close()
+    assert not parent.handle
+
+
 @pytest.mark.parametrize("mode,plan128", [(0, 0), (1, 0), (1, 12)])
 def test_batch_spectrum_views_grouped(gpu, oracle, mode, plan128):
     """Instances whose spectra are consecutive views of one multi-filter
