@@ -217,7 +217,7 @@ class Spectrum:
 
     def __init__(self, N, filters, count=1, on_device=True, stream=None):
         self._parent = None
-        self._nviews, self._close_pending = 0, False
+        self._users, self._close_pending = 0, False  # open views and batches using it
         h = ctypes.c_void_p()
         if on_device:
             ptr = _dev(filters)
@@ -232,18 +232,29 @@ class Spectrum:
     def _wrap(cls, h, N, parent):
         obj = cls.__new__(cls)
         obj._h, obj.N, obj.count, obj._parent = h, N, 1, parent
-        obj._nviews, obj._close_pending = 0, False
+        obj._users, obj._close_pending = 0, False
         return obj
+
+    def _acquire(self):
+        if not self._h or self._close_pending:
+            raise ConvError("the spectrum is closed")
+        self._users += 1
+
+    def _release(self):
+        self._users -= 1
+        if self._close_pending and self._users == 0:
+            self.close()
 
     def view(self, index):
         """Spectrum handle for filter `index` alone, sharing this one's memory
         (cf_spectrum_view); the C handle of this one is kept until its last
         view is closed."""
-        if not self._h or self._close_pending:
-            raise ConvError("cf_spectrum_view: the spectrum is closed")
+        self._acquire()
         h = ctypes.c_void_p()
-        _check(lib().cf_spectrum_view(self._h, index, ctypes.byref(h)), "cf_spectrum_view")
-        self._nviews += 1
+        rc = lib().cf_spectrum_view(self._h, index, ctypes.byref(h))
+        if rc < 0:
+            self._release()
+            _check(rc, "cf_spectrum_view")
         return Spectrum._wrap(h, self.N, self)
 
     @property
@@ -259,20 +270,19 @@ class Spectrum:
 
     def close(self):
         """Destroys the handle.  A view is valid only while its parent lives
-        (cf_spectrum_view), so a spectrum with open views is destroyed when
-        the last of them is closed."""
+        (cf_spectrum_view) and a batch reads its spectra at every launch, so a
+        spectrum with open views or batches is destroyed when the last of them
+        is closed."""
         if not self._h:
             return
-        if self._nviews:
+        if self._users:
             self._close_pending = True
             return
         lib().cf_spectrum_destroy(self._h)
         self._h = ctypes.c_void_p()
         parent, self._parent = self._parent, None
         if parent is not None:
-            parent._nviews -= 1
-            if parent._close_pending and parent._nviews == 0:
-                parent.close()
+            parent._release()
 
     def __del__(self):
         try:
@@ -372,15 +382,20 @@ def autotune(N):
 
 class Batch:
     """A fixed list of (signal, output, Spectrum) convolutions replayed with one
-    graph launch (cf_batch_t).  Keeps references to the spectra it uses."""
+    graph launch (cf_batch_t).  Keeps its signals, outputs and spectra alive
+    until it is closed (a spectrum closed meanwhile is destroyed then)."""
 
     def __init__(self, instances, scale=CF_SCALE_INV_N3, mode=CF_BATCH_PERSISTENT):
+        self._h, self._keep, self._specs, self.mode = ctypes.c_void_p(), list(instances), [], mode
         arr = (cf_instance_t * len(instances))()
         for k, (sig, out, spec) in enumerate(instances):
             arr[k] = cf_instance_t(_dev(sig), _dev(out), spec.handle)
         h = ctypes.c_void_p()
         _check(lib().cf_batch_create(arr, len(instances), scale, mode, ctypes.byref(h)), "cf_batch_create")
-        self._h, self._keep, self.mode = h, list(instances), mode
+        self._h = h
+        for spec in {id(i[2]): i[2] for i in instances}.values():
+            spec._acquire()
+            self._specs.append(spec)
 
     def launch(self, stream=None):
         _check(lib().cf_batch_launch(self._h, _stream(stream)), "cf_batch_launch")
@@ -392,6 +407,9 @@ class Batch:
         if self._h:
             lib().cf_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
+        specs, self._specs = self._specs, []
+        for spec in specs:
+            spec._release()
 
     def __del__(self):
         try:

@@ -175,3 +175,29 @@ def test_spectrum_close_waits_for_its_views(monkeypatch):
     assert destroyed == [1000, 1001, 7] and not parent.handle
     parent.close()  # idempotent
     assert destroyed == [1000, 1001, 7]
+
+
+def test_spectrum_close_waits_for_its_batches(monkeypatch):
+    """A batch reads its spectra at every launch: a spectrum closed while a
+    batch uses it is destroyed when the batch is closed."""
+    calls = []
+
+    class FakeLib:
+        def cf_batch_create(self, arr, count, scale, mode, out):
+            out._obj.value = 55
+            return 0
+
+        def cf_batch_destroy(self, h):
+            calls.append(("batch", h.value))
+
+        def cf_spectrum_destroy(self, h):
+            calls.append(("spectrum", h.value))
+
+    monkeypatch.setattr(api, "lib", lambda: FakeLib())
+    monkeypatch.setattr(api, "_dev", lambda t: 0)
+    spec = api.Spectrum._wrap(ctypes.c_void_p(9), 8, None)
+    b = api.Batch([(None, None, spec), (None, None, spec)])
+    spec.close()
+    assert calls == [] and spec.handle
+    b.close()
+    assert calls == [("batch", 55), ("spectrum", 9)] and not spec.handle
