@@ -239,6 +239,28 @@ def test_full_size_delta_shift_and_roundtrip(gpu, oracle, n):
     torch.cuda.empty_cache()
 
 
+def test_dropin_512_delta_shift(gpu, oracle):
+    """The drop-in call at the largest edge (1 GiB cubes staged through the
+    pinned chunk buffers, the autotuned 512^3 plan): a delta filter at (a, b, c)
+    gives N^3 times the signal circularly shifted by (a, b, c)."""
+    import torch
+    n = 512
+    print("case drop-in KAT delta-shift N=%d" % n, flush=True)
+    sig = oracle.uniform(n ** 3, 17900)
+    filt = np.zeros(n ** 3, np.complex64)
+    a, b, c = 3, n - 7, 200
+    filt[(c * n + b) * n + a] = 1.0
+    out = np.empty_like(sig)
+    r = gpu.fpgaf_conv3D(n, sig, filt, out)
+    assert r.valid and r.sig_pcie_wr_t > 0 and r.sig_pcie_rd_t > 0
+    del filt
+    got = _t(out)
+    want = torch.roll(_t(sig).view(n, n, n), shifts=(c, b, a), dims=(0, 1, 2)).reshape(-1).mul_(float(n ** 3))
+    assert gpu.rel_l2_device(got, want, n ** 3, ref_is_f64=False) <= REL_L2_FP32
+    del got, want, out, sig
+    torch.cuda.empty_cache()
+
+
 def test_rel_l2_device_matches_numpy(gpu, oracle):
     a = oracle.uniform(100003, 1)
     b = (a + np.complex64(1e-3) * oracle.uniform(100003, 2)).astype(np.complex64)
