@@ -321,7 +321,7 @@ int size_res(unsigned n, SizeFns** out) {
 // ---- passes -------------------------------------------------------------------
 struct Pass {
   hipFunction_t f;
-  unsigned tiles, threads;  // tiles per cube (tiled passes) or blocks per cube (others)
+  unsigned tiles, threads;  // work-groups (tiles) per cube, threads per work-group
   const void* in;
   void* out;
   const void* H;
@@ -329,16 +329,14 @@ struct Pass {
   float scale;
   const char* name;
   double bytes;  // algorithmic HBM bytes for the whole launch
-  bool tiled = true;
 };
 
 int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const void* tw) {
   const size_t ce = cube_elems(n);
-  // tiled passes cover the whole batch with one 1-D grid (one work-group per
-  // tile, tile indices in int, grid size in work-items below 2^31); the
-  // others use gridDim.y for the cube
+  // one 1-D grid over the whole batch (one work-group per tile, tile indices
+  // in int), split into launches of whole cubes below 2^31 work-items each
   const unsigned long long per_cube = (unsigned long long)std::max(1u, p.tiles) * p.threads;
-  const unsigned max_cubes = p.tiled ? (unsigned)std::max(1ull, 0x7fffffffull / per_cube) : 65535u;
+  const unsigned max_cubes = (unsigned)std::max(1ull, 0x7fffffffull / per_cube);
   for (unsigned b0 = 0; b0 < nb; b0 += max_cubes) {
     const unsigned cnt = std::min(max_cubes, nb - b0);
     const void* in = (const char*)p.in + b0 * ce * 8;
@@ -349,8 +347,7 @@ int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const voi
     const void* twp = tw;
     int ntotal = (int)(p.tiles * cnt);
     void* args[] = {(void*)&in, (void*)&out, (void*)&H, (void*)&twp, (void*)&hs, (void*)&sc, (void*)&ntotal};
-    const unsigned gx = p.tiled ? (unsigned)ntotal : p.tiles, gy = p.tiled ? 1u : cnt;
-    HIPCHK(hipModuleLaunchKernel(p.f, gx, gy, 1, p.threads, 1, 1, 0, s, args, nullptr));
+    HIPCHK(hipModuleLaunchKernel(p.f, (unsigned)ntotal, 1, 1, p.threads, 1, 1, 0, s, args, nullptr));
   }
   return CF_OK;
 }
