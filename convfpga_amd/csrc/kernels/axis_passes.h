@@ -144,9 +144,8 @@ __device__ __forceinline__ void row_kernel(const float2* in, float2* out, const 
   using K = RowCfg<N>;
   __shared__ float2 lds[K::LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, K::THREADS>(tw_s, tw);
   const RowPass<N, INV, SCALE> k{in, out, tw_s, lds, scale, (int)threadIdx.x % K::T, (int)threadIdx.x / K::T};
-  run_tiles(k, ntotal);
+  run_tiles<N, K::THREADS>(k, ntotal, tw_s, tw);
 }
 
 template <int N, int AXIS, int MODE, bool IN_ZB = false, bool OUT_ZB = false>
@@ -155,10 +154,9 @@ __device__ __forceinline__ void col_kernel(const float2* in, float2* out, const 
   using K = ColCfg<N>;
   __shared__ float2 lds[K::LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, K::THREADS>(tw_s, tw);
   const int c = threadIdx.x % K::C, t = (threadIdx.x / K::C) % K::T, g = threadIdx.x / (K::C * K::T);
   const ColPass<N, AXIS, MODE, IN_ZB, OUT_ZB> k{in, out, H, tw_s, lds, hstride, c, t, g};
-  run_tiles(k, ntotal);
+  run_tiles<N, K::THREADS>(k, ntotal, tw_s, tw);
 }
 
 }  // namespace cfk

@@ -223,23 +223,26 @@ struct RowLds {
   __device__ __forceinline__ float2 ld(int o) const { return p[o + (o >> 4)]; }
 };
 
-// One tile per work-group: K provides Regs, load(tile, Regs&) and
-// process(tile, Regs&).  (A persistent variant that double-buffered the next
-// tile in registers was measured slower on MI355X: the extra registers cost
-// occupancy, and the compiler hoists per-lane addresses out of the tile loop.)
-template <class K>
-__device__ __forceinline__ void run_tiles(const K& k, int ntotal) {
-  const int tile = blockIdx.x;
-  if (tile >= ntotal) return;
-  typename K::Regs a;
-  k.load(tile, a);
-  k.process(tile, a);
-}
-
 template <int N, int THREADS>
 __device__ __forceinline__ void stage_twiddles(float2* tw_s, const float2* __restrict__ tw) {
   for (int i = threadIdx.x; i < N; i += THREADS) tw_s[i] = tw[i];
   __syncthreads();
+}
+
+// One tile per work-group: K provides Regs, load(tile, Regs&) (global memory
+// only) and process(tile, Regs&).  The tile's loads are issued before the
+// twiddle table is staged in LDS, so that the two latencies overlap.  (A
+// persistent variant that double-buffered the next tile in registers was
+// measured slower on MI355X: the extra registers cost occupancy, and the
+// compiler hoists per-lane addresses out of the tile loop.)
+template <int N, int THREADS, class K>
+__device__ __forceinline__ void run_tiles(const K& k, int ntotal, float2* tw_s, const float2* __restrict__ tw) {
+  const int tile = blockIdx.x;
+  if (tile >= ntotal) return;
+  typename K::Regs a;
+  k.load(tile, a);
+  stage_twiddles<N, THREADS>(tw_s, tw);
+  k.process(tile, a);
 }
 
 }  // namespace cfk

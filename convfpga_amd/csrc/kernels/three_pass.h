@@ -109,10 +109,9 @@ __device__ __forceinline__ void plane_kernel(const float2* in, float2* out, cons
   using K = PlaneCfg<N>;
   __shared__ float2 lds[K::LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, K::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
   const PlanePass<N, INV> k{in, out, tw_s, lds, scale, x % K::T, x / K::T, x % N, x / N};
-  run_tiles(k, ntotal);  // tile = plane index over the whole batch
+  run_tiles<N, K::THREADS>(k, ntotal, tw_s, tw);  // tile = plane index over the whole batch
 }
 
 // ---------------------------------------------------------------------------
@@ -256,7 +255,6 @@ __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const 
   static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
   __shared__ float2 lds[K::A_LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, K::A_THREADS>(tw_s, tw);
   const int x = threadIdx.x;
   const int row = x / K::T;
   const int cxl = x % GB, cy2l = (x / GB) % GA, cxh = x / (GB * GA);
@@ -268,9 +266,17 @@ __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const 
   const Xy1Pass<N, GA, GB, INV> k{in, out, tw_s, lds, scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl,
                                   cy2l, nt};
   const int tile = xcd_group_tile<S>((int)blockIdx.x);
-  if (tile >= ntotal) return;
   typename Xy1Pass<N, GA, GB, INV>::Regs a;
+  // At 256^3 the forward pass issues its loads before staging the twiddles
+  // (the two latencies overlap: 54.3 -> 54.0 us, with the middle pass below
+  // 1.3% per convolution); the inverse pass, whose first step multiplies by
+  // twiddles, and the 128^3 passes (one round of work-groups) measured faster
+  // staging them first (128^3 inverse: 8.7 against 9.4 us).
+  constexpr bool EARLY = !INV && N >= 256;
+  if constexpr (!EARLY) stage_twiddles<N, K::A_THREADS>(tw_s, tw);
+  if (tile >= ntotal) return;
   k.load(tile, a);
+  if constexpr (EARLY) stage_twiddles<N, K::A_THREADS>(tw_s, tw);
   k.process(tile, a);
 }
 
@@ -420,15 +426,17 @@ __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const 
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, M::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
   const MidPass<N, GA, GB, GM, HPF> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x,
                                       ntotal <= M::TILES};
   // siblings (consecutive tiles of one x group) on one XCD, back to back
   const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
-  if (tile >= ntotal) return;
   typename MidPass<N, GA, GB, GM, HPF>::Regs a;
+  constexpr bool EARLY = N >= 256;  // loads in flight while the twiddles are staged (see xy1_kernel)
+  if constexpr (!EARLY) stage_twiddles<N, M::THREADS>(tw_s, tw);
+  if (tile >= ntotal) return;
   k.load(tile, a);
+  if constexpr (EARLY) stage_twiddles<N, M::THREADS>(tw_s, tw);
   k.process(tile, a);
 }
 
@@ -441,13 +449,13 @@ __device__ __forceinline__ void mid_spec_kernel(const float2* in, float2* out, c
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[N];
-  stage_twiddles<N, M::THREADS>(tw_s, tw);
   const int x = threadIdx.x;
   const MidPass<N, GA, GB, GM, false> k{in, out, nullptr, tw_s, lds, 0, x % M::LINES, x / M::LINES, x, false};
   const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
   if (tile >= ntotal) return;
   typename MidPass<N, GA, GB, GM, false>::Regs a;
   k.load(tile, a);
+  stage_twiddles<N, M::THREADS>(tw_s, tw);
   k.spectrum(tile, a);
 }
 
