@@ -177,7 +177,10 @@ KERNEL_OF_PASS = {  # kernel of each pass name (N = edge; G, GM = tile-group suf
 
 def kernel_of_pass(name, n, plan_name):
     gm = "_" + plan_name.split("_")[1] if plan_name.startswith("split3_") else ""
-    k = KERNEL_OF_PASS.get(name, "").format(N=n, G=gm.split("m")[0], GM=gm)
+    g = gm.split("m")[0]
+    if name == "xy1_inv" and gm.endswith("o"):  # reads the middle pass's second intermediate: x group GM
+        g = g[:-1] + gm[-2]
+    k = KERNEL_OF_PASS.get(name, "").format(N=n, G=g, GM=gm)
     if plan_name == "axis5z" and name in ("y_fwd", "y_inv", "z_fwd_mul_z_inv"):
         k = k.replace("_%d" % n, "_zb_%d" % n)  # the z-blocked column kernels
     return k

@@ -29,11 +29,12 @@ CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
 CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
 CF_PLAN_SPLIT3_G24M2, CF_PLAN_SPLIT3_G24M1, CF_PLAN_SPLIT3_G28M4, CF_PLAN_SPLIT3_G28M2, CF_PLAN_SPLIT3_G28M1 = 7, 8, 9, 10, 11
 CF_PLAN_AXIS5Z = 12
-CF_NUM_PLANS = 13
+CF_PLAN_SPLIT3_G24M2O, CF_PLAN_SPLIT3_G28M2O = 13, 14
+CF_NUM_PLANS = 15
 CF_BATCH_PERSISTENT, CF_BATCH_GRAPH = 0, 1
 PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44",
               7: "split3_g24m2", 8: "split3_g24m1", 9: "split3_g28m4", 10: "split3_g28m2", 11: "split3_g28m1",
-              12: "axis5z"}
+              12: "axis5z", 13: "split3_g24m2o", 14: "split3_g28m2o"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
 # names of every symbol include/convfpga/*.h declares
@@ -151,11 +152,36 @@ def _host(a, n_elems=None):
     return a.ctypes.data
 
 
-def _dev(x):
+_DEV_DTYPES = {"c64": ("complex64",), "c128": ("complex128",), "f32": ("float32", "complex64")}
+
+
+def _dev(x, nbytes, kind="c64"):
+    """Device address of x for a call that reads or writes `nbytes` bytes of
+    it.  A torch tensor must live on the GPU, be contiguous, hold elements of
+    the expected type and at least `nbytes` bytes; anything else raises before
+    the library is called (a kernel handed a host pointer or a short buffer
+    would fault the GPU).  A plain integer is taken as a raw device address
+    whose extent the caller vouches for."""
     if x is None:
         return None
+    if isinstance(x, bool):
+        raise TypeError("a device buffer is a torch tensor on the GPU or an integer address, not bool")
     if isinstance(x, int):
         return x
+    for attr in ("is_cuda", "is_contiguous", "element_size", "numel", "dtype", "data_ptr"):
+        if not hasattr(x, attr):
+            raise TypeError("a device buffer is a torch tensor on the GPU or an integer address, not %s"
+                            % type(x).__name__)
+    if not x.is_cuda:
+        raise TypeError("device buffer is not on the GPU (device %s)" % (x.device,))
+    if not x.is_contiguous():
+        raise ValueError("device buffer must be contiguous")
+    dt = str(x.dtype).replace("torch.", "")
+    if dt not in _DEV_DTYPES[kind]:
+        raise TypeError("device buffer of dtype %s where %s is expected" % (dt, " or ".join(_DEV_DTYPES[kind])))
+    have = x.numel() * x.element_size()
+    if have < nbytes:
+        raise ValueError("device buffer holds %d bytes, the call needs %d" % (have, nbytes))
     return x.data_ptr()
 
 
@@ -220,7 +246,7 @@ class Spectrum:
         self._users, self._close_pending = 0, False  # open views and batches using it
         h = ctypes.c_void_p()
         if on_device:
-            ptr = _dev(filters)
+            ptr = _dev(filters, 8 * N ** 3 * count)
         else:
             ptr = _host(filters, N ** 3 * count)
         _check(lib().cf_spectrum_create(N, count, ptr, 1 if on_device else 0, _stream(stream), ctypes.byref(h)),
@@ -292,7 +318,9 @@ class Spectrum:
 
 
 def conv3d_device(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
-    _check(lib().cf_conv3d_device(spec.handle, _dev(sig), _dev(out), how_many, scale, precision, _stream(stream)),
+    nb = 8 * spec.N ** 3 * how_many
+    _check(lib().cf_conv3d_device(spec.handle, _dev(sig, nb), _dev(out, nb), how_many, scale, precision,
+                                  _stream(stream)),
            "cf_conv3d_device")
 
 
@@ -309,7 +337,8 @@ def conv3d_host(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_
 
 def conv3d_device_timed(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
     t = cf_pass_times_t()
-    _check(lib().cf_conv3d_device_timed(spec.handle, _dev(sig), _dev(out), how_many, scale, precision,
+    nb = 8 * spec.N ** 3 * how_many
+    _check(lib().cf_conv3d_device_timed(spec.handle, _dev(sig, nb), _dev(out, nb), how_many, scale, precision,
                                         _stream(stream), ctypes.byref(t)), "cf_conv3d_device_timed")
     return t.as_list()
 
@@ -318,19 +347,22 @@ def conv3d_share_timed(spec, sig, out, shares, precision=CF_PREC_FP32, stream=No
     """Per-pass times of one GPU's share (1/shares of every pass's tiles) of a
     slab-decomposed convolution of one cube (cf_conv3d_share_timed)."""
     t = cf_pass_times_t()
-    _check(lib().cf_conv3d_share_timed(spec.handle, _dev(sig), _dev(out), shares, precision, _stream(stream),
+    nb = 8 * spec.N ** 3
+    _check(lib().cf_conv3d_share_timed(spec.handle, _dev(sig, nb), _dev(out, nb), shares, precision, _stream(stream),
                                        ctypes.byref(t)), "cf_conv3d_share_timed")
     return t.as_list()
 
 
 def fft3d_device(N, x, out, how_many=1, inverse=False, stream=None):
-    _check(lib().cf_fft3d_device(N, _dev(x), _dev(out), how_many, 1 if inverse else 0, _stream(stream)),
+    nb = 8 * N ** 3 * how_many
+    _check(lib().cf_fft3d_device(N, _dev(x, nb), _dev(out, nb), how_many, 1 if inverse else 0, _stream(stream)),
            "cf_fft3d_device")
 
 
 def rel_l2_device(test, ref, npts, ref_is_f64, stream=None):
     r = ctypes.c_double()
-    _check(lib().cf_rel_l2_device(_dev(test), _dev(ref), 1 if ref_is_f64 else 0, npts, _stream(stream),
+    refp = _dev(ref, 16 * npts, "c128") if ref_is_f64 else _dev(ref, 8 * npts)
+    _check(lib().cf_rel_l2_device(_dev(test, 8 * npts), refp, 1 if ref_is_f64 else 0, npts, _stream(stream),
                                   ctypes.byref(r)), "cf_rel_l2_device")
     return r.value
 
@@ -349,11 +381,11 @@ def profile_end():
 
 def fill_uniform(x, nfloats, seed, stream=None):
     """Device-side U[-1,1) fill, bit-identical to the host splitmix64 generator."""
-    _check(lib().cf_fill_uniform(_dev(x), nfloats, seed, _stream(stream)), "cf_fill_uniform")
+    _check(lib().cf_fill_uniform(_dev(x, 4 * nfloats, "f32"), nfloats, seed, _stream(stream)), "cf_fill_uniform")
 
 
 def fill_loguniform(x, npts, seed, stream=None):
-    _check(lib().cf_fill_loguniform(_dev(x), npts, seed, _stream(stream)), "cf_fill_loguniform")
+    _check(lib().cf_fill_loguniform(_dev(x, 8 * npts), npts, seed, _stream(stream)), "cf_fill_loguniform")
 
 
 def plan_supported(N, plan):
@@ -389,7 +421,8 @@ class Batch:
         self._h, self._keep, self._specs, self.mode = ctypes.c_void_p(), list(instances), [], mode
         arr = (cf_instance_t * len(instances))()
         for k, (sig, out, spec) in enumerate(instances):
-            arr[k] = cf_instance_t(_dev(sig), _dev(out), spec.handle)
+            nb = 8 * spec.N ** 3
+            arr[k] = cf_instance_t(_dev(sig, nb), _dev(out, nb), spec.handle)
         h = ctypes.c_void_p()
         _check(lib().cf_batch_create(arr, len(instances), scale, mode, ctypes.byref(h)), "cf_batch_create")
         self._h = h

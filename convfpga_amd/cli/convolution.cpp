@@ -18,6 +18,10 @@
 //                     rigour of the CPU-only mode (the reference fixes it when
 //                     it is built, FFTW_PLAN in src/CMakeLists.txt:5-6;
 //                     default estimate as there)
+//   --dump-out <file> after the last iteration, write the filter, the signals
+//                     and the outputs (float32 re/im interleaved, in that
+//                     order: N^3, batch*N^3 and batch*N^3 points) to <file>,
+//                     so that a checker can verify the numbers without FFTW
 //
 // The CPU check and the CPU-only mode follow src/fft_conv3D.cpp:50-266 on
 // FFTW single precision, loaded at run time (libfftw3f.so.3 and its threads
@@ -40,7 +44,7 @@
 namespace {
 
 struct Config {
-  std::string path, wisdomfile = "a.out", fftw_plan = "estimate";
+  std::string path, wisdomfile = "a.out", fftw_plan = "estimate", dump_out;
   unsigned num = 64, iter = 1, threads = 1, batch = 1;
   bool noverify = false, cpuonly = false, usesvm = false, emulate = false;
 };
@@ -283,6 +287,16 @@ void disp_results(const Config& c, const std::vector<fpga_t>& rt) {
   }
 }
 
+// filter (nf points), signals and outputs (ns points each), float32 re/im
+bool dump(const std::string& file, const float2* filter, size_t nf, const float2* sig, const float2* out, size_t ns) {
+  FILE* fp = fopen(file.c_str(), "wb");
+  if (!fp) return false;
+  bool ok = fwrite(filter, sizeof(float2), nf, fp) == nf && fwrite(sig, sizeof(float2), ns, fp) == ns &&
+            fwrite(out, sizeof(float2), ns, fp) == ns;
+  ok = fclose(fp) == 0 && ok;
+  return ok;
+}
+
 void usage(const char* argv0) {
   printf("Usage: %s [options]\n"
          "  -p, --path <file>        code object (default: convfpga_gfx950.hsaco next to this binary)\n"
@@ -296,7 +310,8 @@ void usage(const char* argv0) {
          "  -s, --usesvm             host-mapped calls\n"
          "  -e, --emulate            FPGA emulation (not available)\n"
          "  -h, --help               this text\n"
-         "      --fftw-plan <mode>   estimate | measure | patient | exhaustive (CPU-only mode; default estimate)\n",
+         "      --fftw-plan <mode>   estimate | measure | patient | exhaustive (CPU-only mode; default estimate)\n"
+         "      --dump-out <file>    write filter, signals and outputs (float32 re/im) after the last iteration\n",
          argv0);
 }
 
@@ -322,6 +337,7 @@ int parse_args(int argc, char** argv, Config& c) {
                                 {"emulate", no_argument, nullptr, 'e'},
                                 {"help", no_argument, nullptr, 'h'},
                                 {"fftw-plan", required_argument, nullptr, 'P'},
+                                {"dump-out", required_argument, nullptr, 'D'},
                                 {nullptr, 0, nullptr, 0}};
   int o;
   while ((o = getopt_long(argc, argv, "p:w:n:i:t:yb:cseh", opts, nullptr)) != -1) {
@@ -337,6 +353,7 @@ int parse_args(int argc, char** argv, Config& c) {
       case 'c': c.cpuonly = true; break;
       case 's': c.usesvm = true; break;
       case 'e': c.emulate = true; break;
+      case 'D': c.dump_out = optarg; break;
       case 'h': usage(argv[0]); return 2;
       case 'P': {
         unsigned flag;
@@ -441,6 +458,10 @@ int main(int argc, char** argv) {
     }
   }
   fpga_final();
+  if (rc == EXIT_SUCCESS && !c.dump_out.empty() && !dump(c.dump_out, filter, P, sig, out, P * c.batch)) {
+    fprintf(stderr, "cannot write %s\n", c.dump_out.c_str());
+    rc = EXIT_FAILURE;
+  }
   if (rc == EXIT_SUCCESS) disp_results(c, rt);
   free(sig);
   free(filter);

@@ -75,10 +75,22 @@ struct SizeFns {
   float tune_ms[CF_NUM_PLANS] = {};
 };
 
-bool is_split(int plan) { return plan >= CF_PLAN_SPLIT3_G22 && plan < CF_PLAN_SPLIT3_G22 + cfgeo::kSplitVariants; }
+// four-step plans: CF_PLAN_SPLIT3_G22 .. G28M1 are variants 0..8, then
+// CF_PLAN_SPLIT3_G24M2O and G28M2O variants 9 and 10 (geometry.h)
+bool is_split(int plan) {
+  return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) || plan == CF_PLAN_SPLIT3_G24M2O ||
+         plan == CF_PLAN_SPLIT3_G28M2O;
+}
+int split_variant(int plan) {
+  return plan <= CF_PLAN_SPLIT3_G28M1 ? plan - CF_PLAN_SPLIT3_G22 : 9 + (plan - CF_PLAN_SPLIT3_G24M2O);
+}
+static_assert(cfgeo::kSplitVariants == 11 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
+                  CF_PLAN_SPLIT3_G28M2O == CF_PLAN_SPLIT3_G24M2O + 1 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2O + 1,
+              "plan ids of the four-step variants");
 // plans whose passes go through the workspace (one cube per convolution in flight)
 bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
-int split_variant(int plan) { return plan - CF_PLAN_SPLIT3_G22; }
+// four-step plans whose middle pass writes a second intermediate
+bool two_intermediates(int plan) { return is_split(plan) && cfgeo::split_out_of_place(split_variant(plan)); }
 
 bool plan_ok(unsigned n, int plan) {
   if (!cfgeo::supported((int)n)) return false;
@@ -280,16 +292,17 @@ int size_res(unsigned n, SizeFns** out) {
   for (int vi = 0; vi < cfgeo::kSplitVariants; ++vi) {
     if (!cfgeo::split_supported((int)n, vi)) continue;
     const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
-    char sfx[16];
+    const int gbo = cfgeo::split_gbo(vi);
+    char sfx[16];  // the in-place middle pass of the same input layout and tile (its filter setup too)
     if (gm == gb)
       snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
     else
       snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
     snprintf(name, sizeof name, "cf_xy1_fwd_%u_g%d%d", n, ga, gb);
     CFCHK(get_fn(&s.xy1_fwd[vi], name));
-    snprintf(name, sizeof name, "cf_xy1_inv_%u_g%d%d", n, ga, gb);
+    snprintf(name, sizeof name, "cf_xy1_inv_%u_g%d%d", n, ga, gbo);  // reads the middle pass's output layout
     CFCHK(get_fn(&s.xy1_inv[vi], name));
-    snprintf(name, sizeof name, "cf_mid_%u_%s", n, sfx);
+    snprintf(name, sizeof name, gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
     CFCHK(get_fn(&s.mid[vi], name));
     snprintf(name, sizeof name, "cf_midspec_%u_%s", n, sfx);
     CFCHK(get_fn(&s.mspec[vi], name));
@@ -331,6 +344,23 @@ struct Pass {
   double bytes;  // algorithmic HBM bytes for the whole launch
 };
 
+// The passes of one call: at most CF_MAX_PASSES, held in place (building a
+// schedule touches no heap, so the allocation-free entry point allocates
+// nothing at all).
+struct PassList {
+  Pass p[CF_MAX_PASSES];
+  size_t n = 0;
+  void push_back(const Pass& x) {
+    if (n < CF_MAX_PASSES) p[n++] = x;  // every schedule here has at most 5 passes
+  }
+  size_t size() const { return n; }
+  Pass& operator[](size_t i) { return p[i]; }
+  const Pass& operator[](size_t i) const { return p[i]; }
+  Pass& back() { return p[n - 1]; }
+  const Pass* begin() const { return p; }
+  const Pass* end() const { return p + n; }
+};
+
 int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const void* tw) {
   const size_t ce = cube_elems(n);
   // one 1-D grid over the whole batch (one work-group per tile, tile indices
@@ -353,7 +383,7 @@ int launch_pass(const Pass& p, unsigned n, unsigned nb, hipStream_t s, const voi
 }
 
 // forward 3D FFT: x, y, z passes (in may equal out)
-void fwd_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, std::vector<Pass>& v) {
+void fwd_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, PassList& v) {
   const double io = 2.0 * cube_bytes(n) * nb;
   const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
   const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
@@ -362,7 +392,7 @@ void fwd_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void*
   v.push_back({f.colz_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "z_fwd", io});
 }
 
-void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, float scale, std::vector<Pass>& v) {
+void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void* out, float scale, PassList& v) {
   const double io = 2.0 * cube_bytes(n) * nb;
   const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
   const unsigned ct = cfgeo::col_threads(n), cT = cfgeo::col_tiles(n);
@@ -376,7 +406,7 @@ void inv_passes(const SizeFns& f, unsigned n, unsigned nb, const void* in, void*
 // SPLIT3: sig -> work (pass A), work in place (middle), work -> out (pass C).
 // Returns the index of the first pass after the product (the fpga_t split).
 size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const void* sig, void* out, void* work,
-                   const void* H, long long hstride, float scale, std::vector<Pass>& v,
+                   const void* H, long long hstride, float scale, PassList& v,
                    int precision = CF_PREC_FP32) {
   const double V = (double)cube_bytes(n);
   const double io = 2.0 * V * nb;
@@ -403,10 +433,19 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
     const int vi = split_variant(plan);
     const unsigned at = cfgeo::split_a_threads((int)n, vi), aT = cfgeo::split_a_tiles((int)n, vi);
     const unsigned bt = cfgeo::split_b_threads((int)n, vi), bT = cfgeo::split_b_tiles((int)n, vi);
-    v.push_back({f.xy1_fwd[vi], aT, at, sig, work, nullptr, 0, 1.0f, "xy1_fwd", io});
-    v.push_back({f.mid[vi], bT, bt, work, work, H, hstride, 1.0f, "y2z_fwd_mul_inv", io + hbytes});
+    void* w1 = work;  // first intermediate (the middle pass's input)
+    void* w2 = work;  // second one (its output): the same cube unless the plan writes out of place
+    if (two_intermediates(plan)) {
+      if (sig != out) {
+        w1 = out;  // overwritten by the last pass anyway
+      } else {
+        w2 = (char*)work + (size_t)nb * cube_bytes(n);  // in place: the workspace holds both
+      }
+    }
+    v.push_back({f.xy1_fwd[vi], aT, at, sig, w1, nullptr, 0, 1.0f, "xy1_fwd", io});
+    v.push_back({f.mid[vi], bT, bt, w1, w2, H, hstride, 1.0f, "y2z_fwd_mul_inv", io + hbytes});
     const size_t split = v.size();
-    v.push_back({f.xy1_inv[vi], aT, at, work, out, nullptr, 0, scale, "xy1_inv", io});
+    v.push_back({f.xy1_inv[vi], aT, at, w2, out, nullptr, 0, scale, "xy1_inv", io});
     return split;
   }
   if (plan == CF_PLAN_AXIS5Z) {  // natural -> z-blocked in the workspace -> natural
@@ -427,6 +466,15 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
   return split;
 }
 
+// Workspace bytes conv_passes needs for nb cubes of edge n under `plan`: one
+// intermediate cube per convolution, two for a plan with two intermediates
+// run in place (out of place, the output buffer holds the first).
+size_t work_bytes(int plan, unsigned n, unsigned nb, const void* sig, const void* out) {
+  if (!needs_work(plan)) return 0;
+  const size_t one = cube_bytes(n) * nb;
+  return two_intermediates(plan) && sig == out ? 2 * one : one;
+}
+
 // Filter setup of `plan`: the forward FFT3 of `count` filters into dst, in the
 // layout the plan's product pass reads (in place when src == dst).  Natural
 // order by the axis passes (PLANE3: the plane pass and z); z-blocked for
@@ -435,7 +483,7 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
 // the middle pass straight into the middle pass's spectrum order (4 volume
 // transfers instead of 6 for the axis passes and 2 for a re-layout).
 void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, const void* src, void* dst, void* work,
-                     std::vector<Pass>& v) {
+                     PassList& v) {
   const double io = 2.0 * cube_bytes(n) * count;
   if (plan == CF_PLAN_AXIS5Z) {  // x into the workspace, y into dst (z-blocked), z in place
     const unsigned rt = cfgeo::row_threads(n), rT = cfgeo::row_tiles(n);
@@ -463,7 +511,7 @@ void spectrum_passes(const SizeFns& f, int plan, unsigned n, unsigned count, con
   v.push_back({f.mspec[vi], bT, bt, work, dst, nullptr, 0, 1.0f, "y2z_fwd", io});
 }
 
-int run_passes(const std::vector<Pass>& v, unsigned n, unsigned nb, hipStream_t s, const void* tw,
+int run_passes(const PassList& v, unsigned n, unsigned nb, hipStream_t s, const void* tw,
                hipEvent_t* ev_before /* nullable, v.size()+1 events */) {
   if (ev_before) HIPCHK(hipEventRecord(ev_before[0], s));
   for (size_t i = 0; i < v.size(); ++i) {
@@ -672,7 +720,9 @@ int autotune(unsigned n, SizeFns* f) {
     cleanup();
     return fail(CF_ERR_NOMEM, "autotune scratch allocation failed");
   }
-  rc = acquire_work(V, s, &work);
+  size_t wneed = 0;  // every candidate runs out of place on the scratch cubes
+  for (int p : cand) wneed = std::max(wneed, work_bytes(p, n, 1, sig, out));
+  rc = acquire_work(std::max(wneed, V), s, &work);
   long long nf = (long long)(V / 4);
   unsigned long long seed = 12345;
   void* fa[] = {&sig, &nf, &seed};
@@ -685,7 +735,7 @@ int autotune(unsigned n, SizeFns* f) {
   for (int round = 0; round < 2 && rc == CF_OK; ++round) {
     for (int p : cand) {
       if (rc != CF_OK) break;
-      std::vector<Pass> v;
+      PassList v;
       conv_passes(*f, p, n, 1, sig, out, work, H, 0, 1.0f, v);
       for (int w = 0; w < 2 && rc == CF_OK; ++w) rc = run_passes(v, n, 1, s, f->tw, nullptr);
       if (rc == CF_OK && hipEventRecord(g.ev[0], s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "event");
@@ -936,11 +986,11 @@ int pipe_feed(const PipeJob& j, PipeSync& ps, std::vector<hipEvent_t>& tev) {
     if (hipEventRecord(gp.ev_loaded[sl], gp.s_in) != hipSuccess ||
         hipStreamWaitEvent(j.s, gp.ev_loaded[sl], 0) != hipSuccess)
       return stop(fail(CF_ERR_RUNTIME, "upload event of group %u", k));
-    std::vector<Pass> p;
+    PassList p;
     const void* H = (const char*)j.H + (size_t)k0 * j.hstride * 8;
     void* work = nullptr;
     if (needs_work(j.plan)) {
-      const int rc = acquire_work(slot_bytes, j.s, &work);
+      const int rc = acquire_work(work_bytes(j.plan, j.n, j.group, slot, slot), j.s, &work);
       if (rc != CF_OK) return stop(rc);
     }
     conv_passes(*j.f, j.plan, j.n, nb, slot, slot, work, H, j.hstride, j.scale, p, j.precision);
@@ -1077,7 +1127,7 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   hipEvent_t* ev = g.ev;
   hipStream_t s = g.stream;
   CFCHK(ensure_device(&g.d_filt, &g.d_filt_bytes, V));
-  std::vector<Pass> filt_p, conv_p;
+  PassList filt_p, conv_p;
   if (mode == CFE_MODE_SVM_BATCH) {
     // filter upload and transform, then the cubes stream through the
     // overlapped pipeline; times as the reference's batch call reports them
@@ -1101,7 +1151,11 @@ extern "C" int cfe_conv_host(unsigned n, const void* sig, const void* filter, vo
   }
   CFCHK(ensure_device(&g.d_data, &g.d_data_bytes, SV));
   void* work = nullptr;
-  if (needs_work(plan)) CFCHK(acquire_work(SV, s, &work));
+  // the copying call runs in place in d_data; the _svm call reads the mapped
+  // signal and writes d_data, then the last pass writes the mapped output
+  if (needs_work(plan))
+    CFCHK(acquire_work(std::max(V, work_bytes(plan, n, how_many, g.d_data, mode == CFE_MODE_COPY ? g.d_data : nullptr)),
+                       s, &work));
 
   if (mode == CFE_MODE_COPY) {
     // Blocking phases as in the reference (convfpga.c:75-375); each host
@@ -1214,7 +1268,7 @@ extern "C" int cf_spectrum_create(unsigned n, unsigned count, const void* filter
   void* work = nullptr;
   if (rc == CF_OK && needs_work(h->plan)) rc = acquire_work(bytes, s, &work);
   if (rc == CF_OK) {
-    std::vector<Pass> p;
+    PassList p;
     spectrum_passes(*f, h->plan, n, count, src, h->d, work, p);
     rc = run_passes(p, n, count, s, f->tw, nullptr);
     if (rc == CF_OK && work) rc = release_work(s);
@@ -1318,8 +1372,8 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
   const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
   hipStream_t s = (hipStream_t)stream;
   void* work = nullptr;
-  if (needs_work(h->plan)) CFCHK(acquire_work(cube_bytes(n) * how_many, s, &work));
-  std::vector<Pass> p;
+  if (needs_work(h->plan)) CFCHK(acquire_work(work_bytes(h->plan, n, how_many, d_sig, d_out), s, &work));
+  PassList p;
   conv_passes(*f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p, precision);
   struct WorkRelease {  // the workspace is marked busy on s whatever path returns
     void* w;
@@ -1406,7 +1460,7 @@ extern "C" int cf_conv3d_share_timed(cf_spectrum_t h, const void* d_sig, void* d
   const unsigned n = h->n;
   hipStream_t s = (hipStream_t)stream;
   void* work = nullptr;
-  if (needs_work(h->plan)) CFCHK(acquire_work(cube_bytes(n), s, &work));
+  if (needs_work(h->plan)) CFCHK(acquire_work(work_bytes(h->plan, n, 1, d_sig, d_out), s, &work));
   struct WorkRelease {
     void* w;
     hipStream_t s;
@@ -1414,7 +1468,7 @@ extern "C" int cf_conv3d_share_timed(cf_spectrum_t h, const void* d_sig, void* d
       if (w) release_work(s);
     }
   } wr{work, s};
-  std::vector<Pass> p;
+  PassList p;
   conv_passes(*f, h->plan, n, 1, d_sig, d_out, work, h->d, 0, (float)(1.0 / (double)cube_elems(n)), p, precision);
   // every work-group count stays a multiple of 8 x the widest sibling group (8)
   for (const Pass& q : p)
@@ -1493,7 +1547,7 @@ extern "C" int cf_fft3d_device(unsigned n, const void* d_in, void* d_out, unsign
   CFCHK(require_ready());
   SizeFns* f = nullptr;
   CFCHK(size_res(n, &f));
-  std::vector<Pass> p;
+  PassList p;
   if (inverse)
     inv_passes(*f, n, how_many, d_in, d_out, 1.0f, p);
   else
@@ -1605,6 +1659,35 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
                     "create it under CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
     }
   }
+  if (mode == CF_BATCH_PERSISTENT) {
+    // The passes of different instances run at the same time in one launch,
+    // so no instance may write memory that another one reads or writes: every
+    // output must be disjoint from every other instance's signal and output
+    // (an output may be its own signal; a chained pipeline needs graph mode,
+    // which keeps the instance order).
+    struct Span {
+      const char* a;
+      const char* b;
+    };
+    auto span = [&](const void* p, unsigned i) {
+      return Span{(const char*)p, (const char*)p + cube_bytes(inst[i].h->n)};
+    };
+    auto meet = [](Span x, Span y) { return x.a < y.b && y.a < x.b; };
+    for (unsigned i = 0; i < count; ++i) {
+      const Span si = span(inst[i].d_sig, i), oi = span(inst[i].d_out, i);
+      if (meet(si, oi) && si.a != oi.a)
+        return fail(CF_ERR_ARG, "instance %u: output partly overlaps its own signal", i);
+      for (unsigned j = 0; j < count; ++j) {
+        if (j == i) continue;
+        const Span sj = span(inst[j].d_sig, j), oj = span(inst[j].d_out, j);
+        if (meet(oi, sj) || meet(oi, oj))
+          return fail(CF_ERR_ARG,
+                      "instances %u and %u: the output of %u overlaps the buffers of %u; instances of a "
+                      "persistent batch run concurrently (use CF_BATCH_GRAPH for chained convolutions)",
+                      i, j, i, j);
+      }
+    }
+  }
   if (!g.capture) HIPCHK(hipStreamCreateWithFlags(&g.capture, hipStreamNonBlocking));
   auto* b = new cf_batch_s;
   b->mode = mode;
@@ -1696,7 +1779,7 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
              (const char*)inst[i + run].d_sig == (const char*)inst[i].d_sig + run * V &&
              (char*)inst[i + run].d_out == (char*)inst[i].d_out + run * V)
         ++run;
-      if (needs_work(inst[i].h->plan)) work_need = std::max(work_need, (size_t)run * V);
+      work_need = std::max(work_need, work_bytes(inst[i].h->plan, n, run, inst[i].d_sig, inst[i].d_out));
       runs.push_back({i, run, shared});
       i += run;
     }
@@ -1713,7 +1796,7 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
       const unsigned i = r.i, n = inst[i].h->n;
       const long long hstride = (r.run > 1 && !r.shared) ? (long long)cube_elems(n) : 0;
       const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
-      std::vector<Pass> p;
+      PassList p;
       conv_passes(*fns[i], inst[i].h->plan, n, r.run, inst[i].d_sig, inst[i].d_out, b->d_work, inst[i].h->d, hstride,
                   scale, p);
       rc = run_passes(p, n, r.run, g.capture, fns[i]->tw, nullptr);

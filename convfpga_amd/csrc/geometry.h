@@ -34,14 +34,23 @@ constexpr int plane_tiles(int n) { return n; }
 
 // 3-pass four-step schedule (N = 128 and 256):
 // y = (N/16)*y1 + y2 with 16 values of y1 and N/16 of y2.  Variant v: GA = y2
-// values per outer-pass tile, GB = x group of the intermediate layout (runs
-// of GA*GB points), GM = x values per middle-pass tile (GM divides GB).
-constexpr int kSplitVariants = 9;
-constexpr int split_ga(int v) { return v == 2 || v == 3 ? 4 : 2; }
-constexpr int split_gb(int v) { return v == 0 || v == 2 ? 2 : v <= 5 ? 4 : 8; }
-constexpr int split_gm(int v) {
-  return v == 0 || v == 2 || v == 4 || v == 7 ? 2 : v == 1 || v == 3 || v == 6 ? 4 : 1;
-}
+// values per outer-pass tile, GB = x group of the intermediate the first pass
+// writes (runs of GA*GB points), GM = x values per middle-pass tile (GM
+// divides GB), GBO = x group of the intermediate the middle pass writes and
+// the last pass reads: GB (in place) or GM (out of place, every tile's output
+// one contiguous block).
+constexpr int kSplitVariants = 11;
+//                                      g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o
+constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2};
+constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8};
+constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2};
+constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2};
+constexpr int split_ga(int v) { return kSplitGA[v]; }
+constexpr int split_gb(int v) { return kSplitGB[v]; }
+constexpr int split_gm(int v) { return kSplitGM[v]; }
+constexpr int split_gbo(int v) { return kSplitGBO[v]; }
+// the middle pass writes a second intermediate (the last pass reads that one)
+constexpr bool split_out_of_place(int v) { return kSplitGBO[v] != kSplitGB[v]; }
 constexpr bool split_supported(int n, int v) { return (n == 128 || n == 256) && v >= 0 && v < kSplitVariants; }
 constexpr int split_a_threads(int n, int v) { return n * split_ga(v); }
 constexpr int split_a_tiles(int n, int v) { return n * ((n / 16) / split_ga(v)); }

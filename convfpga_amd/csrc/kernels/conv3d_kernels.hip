@@ -111,6 +111,11 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_midspec_##N##_##SUFFIX(CF_ARGS) { \
     cfk::mid_spec_kernel<N, GA, GB, GM>(in, out, tw, ntotal);                                                  \
   }
+// middle pass that writes the out-of-place intermediate of x group GBO (= GM)
+#define CF_SPLIT_BO(N, GA, GB, GM, GBO, SUFFIX)                                                                 \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_mid_##N##_##SUFFIX(CF_ARGS) { \
+    cfk::mid_kernel<N, GA, GB, GM, (GM <= 2), GBO>(in, out, H, tw, hstride, ntotal);                            \
+  }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
 // (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
@@ -128,6 +133,8 @@ CF_SPLIT_B(256, 2, 4, 1, g24m1)
 CF_SPLIT_B(256, 2, 8, 4, g28m4)
 CF_SPLIT_B(256, 2, 8, 2, g28m2)
 CF_SPLIT_B(256, 2, 8, 1, g28m1)
+CF_SPLIT_BO(256, 2, 4, 2, 2, g24m2o)
+CF_SPLIT_BO(256, 2, 8, 2, 2, g28m2o)
 CF_SPLIT_AC(128, 2, 2)
 CF_SPLIT_AC(128, 2, 4)
 CF_SPLIT_AC(128, 4, 2)
@@ -142,6 +149,8 @@ CF_SPLIT_B(128, 2, 4, 1, g24m1)
 CF_SPLIT_B(128, 2, 8, 4, g28m4)
 CF_SPLIT_B(128, 2, 8, 2, g28m2)
 CF_SPLIT_B(128, 2, 8, 1, g28m1)
+CF_SPLIT_BO(128, 2, 4, 2, 2, g24m2o)
+CF_SPLIT_BO(128, 2, 8, 2, 2, g28m2o)
 template <int N, int V>
 struct SplitGeoCheck {
   using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;
@@ -151,14 +160,15 @@ struct SplitGeoCheck {
                 "split geometry");
   static constexpr bool ok = true;
 };
-static_assert(SplitGeoCheck<256, 0>::ok && SplitGeoCheck<256, 1>::ok && SplitGeoCheck<256, 2>::ok &&
-                  SplitGeoCheck<256, 3>::ok && SplitGeoCheck<256, 4>::ok && SplitGeoCheck<256, 5>::ok &&
-                  SplitGeoCheck<256, 6>::ok && SplitGeoCheck<256, 7>::ok && SplitGeoCheck<256, 8>::ok &&
-                  SplitGeoCheck<128, 0>::ok && SplitGeoCheck<128, 1>::ok && SplitGeoCheck<128, 2>::ok &&
-                  SplitGeoCheck<128, 3>::ok && SplitGeoCheck<128, 4>::ok && SplitGeoCheck<128, 5>::ok &&
-                  SplitGeoCheck<128, 6>::ok && SplitGeoCheck<128, 7>::ok && SplitGeoCheck<128, 8>::ok &&
-                  cfgeo::kSplitVariants == 9,
-              "");
+template <int N, int V = 0>
+struct SplitGeoCheckAll {
+  static constexpr bool ok = SplitGeoCheck<N, V>::ok && SplitGeoCheckAll<N, V + 1>::ok;
+};
+template <int N>
+struct SplitGeoCheckAll<N, cfgeo::kSplitVariants> {
+  static constexpr bool ok = true;
+};
+static_assert(SplitGeoCheckAll<256>::ok && SplitGeoCheckAll<128>::ok, "split geometry");
 
 // ---- split-FP16 MFMA form of the four-step passes (split_mfma.h) ----------------------
 // outer passes GA = 2 over an intermediate of x group 4, middle tiles of 2 x:
@@ -308,4 +318,5 @@ extern "C" __global__ __launch_bounds__(256) void cf_fill_loguniform(float2* out
     out[p] = make_float2((float)(mag * c), (float)(mag * s));
   }
 }
+
 

@@ -304,11 +304,17 @@ struct MidGeoN {
 template <int GB, int GM>
 using MidGeo = MidGeoN<256, GB, GM>;
 
-// Middle pass on tile (k1, x0 .. x0+GM-1).
-template <int N, int GA, int GB, int GM, bool HPF = false>
+// Middle pass on tile (k1, x0 .. x0+GM-1): reads the intermediate of x group
+// GB, writes one of x group GBO.  GBO = GB: in place.  GBO = GM (out of
+// place): the tile's output is one contiguous block of N*Y2*GM points, so it
+// stores whole lines however narrow the tile (the reads stay short runs,
+// shared with the sibling tiles through L2).
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB>
 struct MidPass {
   using K = SplitGeo<N, GA, GB>;
+  using KO = SplitGeo<N, GA, GBO>;
   using M = MidGeoN<N, GB, GM>;
+  static_assert(GBO == GB || GBO == GM, "the output layout is the input's or the tile's own block");
   static constexpr int L = M::LINES, Y2 = M::Y2, T = M::T, RPT = M::RPT;
   struct Regs {
     float2 v[16];
@@ -327,7 +333,8 @@ struct MidPass {
   // Buffer accesses (a 32-bit offset per thread, the strides of the loops in
   // the scalar offset) instead of a 64-bit address per access.
   static constexpr unsigned CUBE_BYTES = (unsigned)((size_t)N * N * N * sizeof(float2));
-  static constexpr unsigned M_STEP = T * Y2 * GB * sizeof(float2);  // intermediate: z += T
+  static constexpr unsigned M_STEP = T * Y2 * GB * sizeof(float2);    // input intermediate: z += T
+  static constexpr unsigned M_STEP_O = T * Y2 * GBO * sizeof(float2);  // output intermediate: z += T
   static constexpr unsigned K2_STEP = N * GM * sizeof(float2);      // spectrum block: k2 += 1
   __device__ __forceinline__ __amdgpu_buffer_rsrc_t cube_rsrc(const float2* base, int tile) const {
     return buffer_rsrc(base + (size_t)(tile / M::TILES) * N * N * N, CUBE_BYTES);
@@ -337,6 +344,11 @@ struct MidPass {
     const int tt = tile % M::TILES;
     const int x0 = (tt % M::NXM) * GM;
     return (unsigned)(K::mid_index(x0 + l % GM, tt / M::NXM, l / GM, zt) * sizeof(float2));
+  }
+  __device__ __forceinline__ unsigned thread_voff_out(int tile) const {
+    const int tt = tile % M::TILES;
+    const int x0 = (tt % M::NXM) * GM;
+    return (unsigned)(KO::mid_index(x0 + l % GM, tt / M::NXM, l / GM, zt) * sizeof(float2));
   }
   // the Y2 spectrum values of row `row` of the tile: [k1][x/GM][k2][kz][x%GM]
   __device__ __forceinline__ void load_h(int tile, int row, float2* hv) const {
@@ -400,9 +412,9 @@ struct MidPass {
     __syncthreads();
     fft_line<N, true>(r.v, zt, view, tw_s);
     const __amdgpu_buffer_rsrc_t rs = cube_rsrc(out, tile);
-    const unsigned voff = thread_voff(tile);
+    const unsigned voff = thread_voff_out(tile);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) buffer_st2(rs, voff, m * M_STEP, r.v[m]);
+    for (int m = 0; m < 16; ++m) buffer_st2(rs, voff, m * M_STEP_O, r.v[m]);
   }
   // Filter setup: the forward half only (z FFT, Y2-point DFT over y2), whose
   // result is the tile's block of the full 3D spectrum; it is stored where
@@ -429,19 +441,19 @@ struct MidPass {
   }
 };
 
-template <int N, int GA, int GB, int GM, bool HPF = false>
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
   using M = MidGeoN<N, GB, GM>;
+  using P = MidPass<N, GA, GB, GM, HPF, GBO>;
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[N];
   const int x = threadIdx.x;
-  const MidPass<N, GA, GB, GM, HPF> k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x,
-                                      ntotal <= M::TILES};
+  const P k{in, out, H, tw_s, lds, hstride, x % M::LINES, x / M::LINES, x, ntotal <= M::TILES};
   // siblings (consecutive tiles of one x group) on one XCD, back to back
   const int tile = xcd_group_tile<M::SIBLINGS>((int)blockIdx.x);
-  typename MidPass<N, GA, GB, GM, HPF>::Regs a;
+  typename P::Regs a;
   constexpr bool EARLY = N >= 256;  // loads in flight while the twiddles are staged (see xy1_kernel)
   if constexpr (!EARLY) stage_twiddles<N, M::THREADS>(tw_s, tw);
   if (tile >= ntotal) return;

@@ -37,7 +37,8 @@ extern "C" {
                                * accumulation, one power-of-two scale per tile.
                                * N = 256 with a spectrum of a four-step plan
                                * whose middle tiles are 2 x wide (split3_g22,
-                               * g42, g24m2, g28m2); other spectra: CF_ERR_ARG */
+                               * g42, g24m2, g28m2, g24m2o, g28m2o); other
+                               * spectra: CF_ERR_ARG */
 
 /* execution plans (which passes and tiles a convolution of edge N runs) */
 #define CF_PLAN_AUTO 0       /* the autotuner's choice (default)                              */
@@ -56,7 +57,15 @@ extern "C" {
 #define CF_PLAN_SPLIT3_G28M1 11
 #define CF_PLAN_AXIS5Z 12 /* the five passes with a z-blocked intermediate in the workspace: the
                              z pass reads and writes contiguous blocks; N in {64,...,512}   */
-#define CF_NUM_PLANS 13
+/* four-step plans whose middle pass writes a second intermediate, laid out so
+ * that every middle tile (2 x wide) stores one contiguous block and the last
+ * pass reads runs of 4 points: the first pass writes runs of 2*4 (G24M2O) or
+ * 2*8 (G28M2O) points.  The second intermediate needs one more cube of
+ * workspace when the output is the signal (in place); otherwise the output
+ * buffer holds the first one.  Spectrum layout of the other GM = 2 plans. */
+#define CF_PLAN_SPLIT3_G24M2O 13
+#define CF_PLAN_SPLIT3_G28M2O 14
+#define CF_NUM_PLANS 15
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
@@ -164,7 +173,12 @@ int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
  *     (CF_PLAN_AXIS5 / CF_PLAN_PLANE3).
  *   CF_BATCH_GRAPH: the passes of each instance's own plan, recorded in order
  *     into one graph (any edge and plan).
- * Outputs may alias their signals.  Concurrent launches of one batch race. */
+ * An output may alias its own signal.  In CF_BATCH_PERSISTENT mode the
+ * instances run concurrently, so each output must be disjoint from every
+ * other instance's signal and output (CF_ERR_ARG otherwise); graph mode keeps
+ * the given order, so chained instances (one reads another's output) and
+ * shared outputs (the last instance wins) are allowed there.  Concurrent
+ * launches of one batch race. */
 typedef struct {
   const void *d_sig;
   void *d_out;

@@ -348,76 +348,17 @@ double orc_rel_l2_f32_f32(const float *test, const float *ref, size_t npts) {
   return den > 0.0 ? sqrt(num / den) : sqrt(num);
 }
 
-/* ------------------------------------------------ optional FFTW via dlopen */
-typedef void *fftwf_plan_t;
-typedef fftwf_plan_t (*p_plan_many)(int, const int *, int, float (*)[2], const int *, int, int,
-                                    float (*)[2], const int *, int, int, int, unsigned);
-typedef void (*p_exec)(fftwf_plan_t);
-typedef void (*p_destroy)(fftwf_plan_t);
-typedef int (*p_init_threads)(void);
-typedef void (*p_plan_nthreads)(int);
-
-static struct {
-  int tried, ok;
-  p_plan_many plan_many;
-  p_exec exec;
-  p_destroy destroy;
-  p_init_threads init_threads;
-  p_plan_nthreads plan_nthreads;
-} g_fftw;
-
-static void orc_fftw_load(void) {
-  if (g_fftw.tried) return;
-  g_fftw.tried = 1;
-  void *lib = dlopen("libfftw3f.so.3", RTLD_NOW | RTLD_GLOBAL);
-  if (!lib) return;
-  g_fftw.plan_many = (p_plan_many)dlsym(lib, "fftwf_plan_many_dft");
-  g_fftw.exec = (p_exec)dlsym(lib, "fftwf_execute");
-  g_fftw.destroy = (p_destroy)dlsym(lib, "fftwf_destroy_plan");
-  void *thr = dlopen("libfftw3f_threads.so.3", RTLD_NOW | RTLD_GLOBAL);
-  if (!thr) thr = dlopen("libfftw3f_omp.so.3", RTLD_NOW | RTLD_GLOBAL);
-  if (thr) {
-    g_fftw.init_threads = (p_init_threads)dlsym(thr, "fftwf_init_threads");
-    g_fftw.plan_nthreads = (p_plan_nthreads)dlsym(thr, "fftwf_plan_with_nthreads");
-  }
-  g_fftw.ok = g_fftw.plan_many && g_fftw.exec && g_fftw.destroy;
-}
-
-int orc_fftw_available(void) { orc_fftw_load(); return g_fftw.ok; }
-
-int orc_conv3d_fftw(int n, int batch, const float *sig, const float *filt, float *out, int nthreads) {
-  orc_fftw_load();
-  if (!g_fftw.ok) return -1;
-  if (!orc_is_pow2(n) || batch < 1 || !sig || !filt || !out) return -1;
-  const unsigned FFTW_ESTIMATE_FLAG = 1u << 6;
-  const size_t p = (size_t)n * n * n;
-  float (*h)[2] = (float (*)[2])malloc(sizeof(float) * 2 * p);
-  if (!h) return -2;
-  if (g_fftw.init_threads && g_fftw.plan_nthreads && g_fftw.init_threads())
-    g_fftw.plan_nthreads(nthreads > 0 ? nthreads : orc_max_threads());
-  const int dims[3] = {n, n, n};
-  float (*o)[2] = (float (*)[2])out;
-  fftwf_plan_t pf = g_fftw.plan_many(3, dims, 1, h, NULL, 1, (int)p, h, NULL, 1, (int)p, -1, FFTW_ESTIMATE_FLAG);
-  fftwf_plan_t ps = g_fftw.plan_many(3, dims, batch, o, NULL, 1, (int)p, o, NULL, 1, (int)p, -1, FFTW_ESTIMATE_FLAG);
-  fftwf_plan_t pi = g_fftw.plan_many(3, dims, batch, o, NULL, 1, (int)p, o, NULL, 1, (int)p, +1, FFTW_ESTIMATE_FLAG);
-  memcpy(h, filt, sizeof(float) * 2 * p);
-  memcpy(out, sig, sizeof(float) * 2 * p * (size_t)batch);
-  g_fftw.exec(pf);
-  g_fftw.exec(ps);
-  for (int b = 0; b < batch; ++b) {
-    #pragma omp parallel for
-    for (long long i = 0; i < (long long)p; ++i) {
-      float *s = o[(size_t)b * p + i];
-      float tr = (s[0] * h[i][0]) - (s[1] * h[i][1]);
-      float ti = (s[0] * h[i][1]) + (s[1] * h[i][0]);
-      s[0] = tr;
-      s[1] = ti;
+/* ------------------------------------------------ FFTW availability probe */
+/* The CPU baseline's label names whether FFTW3f was present; the checker never
+ * routes through it (its single-precision output is not reproducible here). */
+int orc_fftw_available(void) {
+  static int tried = 0, ok = 0;
+  if (!tried) {
+    tried = 1;
+    void *lib = dlopen("libfftw3f.so.3", RTLD_NOW | RTLD_LOCAL);
+    if (lib) {
+      ok = dlsym(lib, "fftwf_plan_many_dft") && dlsym(lib, "fftwf_execute") && dlsym(lib, "fftwf_destroy_plan");
     }
   }
-  g_fftw.exec(pi);
-  g_fftw.destroy(pf);
-  g_fftw.destroy(ps);
-  g_fftw.destroy(pi);
-  free(h);
-  return 0;
+  return ok;
 }

@@ -25,10 +25,25 @@
  *                           346-348; src/fft_conv3D.cpp:201-203, 233).
  *   - orc_conv3d_brute_f64: O(P^2) direct circular convolution (definition).
  *
- * Pinning (see tests/test_oracle.py): numpy pocketfft golden fixtures
- * (tests/golden), brute force for N<=16, delta/shift/tone closed forms,
- * Parseval, linearity, round trip.  FFTW's own fp32 output is not available
- * (FFTW3 is absent): parity at the FFTW boundary is pinned only statistically.
+ * Pinning (tests/test_oracle.py), every function by a check that a plausible
+ * mistake fails:
+ *   orc_fft3d_f64         numpy pocketfft golden fixtures and live numpy,
+ *                         single-tone KAT (sign -, amplitude P), Parseval,
+ *                         linearity, round trip
+ *   orc_conv3d_f64        O(P^2) brute force (N <= 16), delta / shifted-delta
+ *                         closed forms, golden fixtures
+ *   orc_conv3d_brute_f64  numpy index-arithmetic restatement of the definition
+ *   orc_fft3d_f32         FP64 transform and numpy (1e-9 < rel-L2 < 1e-6, both
+ *                         signs), single-tone KAT in both directions, round
+ *                         trip, thread-count independence
+ *   orc_conv3d_f32        FP64 oracle (FP32-rounding-sized error), batch
+ *                         instances equal single calls
+ *   orc_conv3d_f32_spec   bit-equal to orc_conv3d_f32 with H = orc_fft3d_f32(filter)
+ *   orc_snr_db, orc_rel_l2_*  numpy norms and closed forms
+ *   orc_fill_*            numpy port (bit-exact / one ulp), glibc rand()
+ *   orc_fftw_available, orc_max_threads  direct dlopen probe, OpenMP default
+ * FFTW's own fp32 output is not available (FFTW3 is absent): parity at the
+ * FFTW boundary is pinned only statistically.
  *
  * Data layout everywhere: interleaved complex {re, im}, linear index
  * z*N*N + y*N + x with x fastest (host/include/convfpga/convfpga.h:16-19).
@@ -75,11 +90,10 @@ double orc_snr_db(const float *ref, const float *test, size_t npts);
 double orc_rel_l2_f32_f64(const float *test, const double *ref, size_t npts);
 double orc_rel_l2_f32_f32(const float *test, const float *ref, size_t npts);
 
-/* ---- optional FFTW3f (dlopen at run time; absent in this image) ---- */
+/* ---- FFTW3f probe (dlopen at run time; absent in this image) ---- */
+/* 1 when libfftw3f.so.3 can be opened and exports the plan/execute/destroy
+ * entry points: the CPU-baseline label says whether FFTW was present. */
 int orc_fftw_available(void);
-/* same contract as orc_conv3d_f32 but through FFTW (FFTW_ESTIMATE, in-place,
- * plan_many_dft rank 3), when libfftw3f.so.3 can be opened.  Returns -1 otherwise. */
-int orc_conv3d_fftw(int n, int batch, const float *sig, const float *filt, float *out, int nthreads);
 
 int orc_max_threads(void);
 

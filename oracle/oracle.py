@@ -46,7 +46,6 @@ def lib():
             "orc_rel_l2_f32_f64": (d, [f32p, f64p, sz]),
             "orc_rel_l2_f32_f32": (d, [f32p, f32p, sz]),
             "orc_fftw_available": (i, []),
-            "orc_conv3d_fftw": (i, [i, i, f32p, f32p, f32p, i]),
             "orc_max_threads": (i, []),
         }
         for name, (res, args) in sig.items():
@@ -138,13 +137,6 @@ def rel_l2(test, ref):
 
 def fftw_available():
     return bool(lib().orc_fftw_available())
-
-
-def conv3d_fftw(n, sig, filt, batch=1, nthreads=0):
-    out = np.empty(n ** 3 * batch, np.complex64)
-    if lib().orc_conv3d_fftw(n, batch, _f32(sig), _f32(filt), _f32(out), nthreads) != 0:
-        raise RuntimeError("FFTW unavailable")
-    return out
 
 
 def max_threads():

@@ -50,3 +50,41 @@ def test_batch_line_checks_every_instance():
     worst = d["rel_l2_vs_fp64_max_by_mode"]
     assert set(worst) == set(d["modes_ms_per_batch"])
     assert all(math.isfinite(v) and v <= 1e-5 for v in worst.values()), worst
+
+
+@pytest.mark.gpu
+def test_c4_mixed_line_checks_all_29_instances():
+    """The mixed-size workload at its stated composition (16x32^3 + 8x64^3 +
+    4x128^3 + 1x256^3, own filters, shuffled): every instance of every batch
+    mode within 1e-5 of the FP64 oracle."""
+    print("case bench c4_mixed", flush=True)
+    r = _bench("--workload", "c4_mixed", "--steps", "3", "--warmup", "1", "--cpu-reps", "1")
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["config"]["edges"] == {"32": 16, "64": 8, "128": 4, "256": 1}
+    assert d["config"]["instances"] == 29 and d["instances_checked"] == 29
+    worst = d["rel_l2_vs_fp64_max_by_mode"]
+    assert set(worst) == set(d["modes_ms_per_batch"]) and len(worst) >= 2
+    assert all(math.isfinite(v) and v <= 1e-5 for v in worst.values()), worst
+    assert d["value"] > 0 and d["roofline"]["bound"] == "hbm"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_c5_split_line_checks_every_instance():
+    """4x256^3 + 1x512^3 with own filters, for U[-1,1) and log-uniform
+    inputs: every instance of both forms (FP32 and split-FP16 MFMA) within the
+    stated bound of 1e-5 against the FP64 oracle, 512^3 included."""
+    print("case bench c5_split", flush=True)
+    r = _bench("--workload", "c5_split", "--steps", "2", "--warmup", "1", "--cpu-reps", "1", timeout=800)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    acc = d["rel_l2_vs_fp64"]
+    assert set(acc) == {"uniform", "loguniform"}
+    for dist, e in acc.items():
+        assert e["instances"][:4] == ["256^3"] * 4 and e["instances"][4].startswith("512^3")
+        for form in ("fp32", "split_fp16_mfma"):
+            errs = e[form]
+            assert len(errs) == 5, (dist, form)
+            assert all(math.isfinite(v) and 0 < v <= 1e-5 for v in errs), (dist, form, errs)
+    assert d["value"] > 0 and d["cpu_baseline"]["value"] > 0

@@ -117,11 +117,13 @@ def test_plan_support_table():
     for n in C.SUPPORTED_SIZES:
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5) == 1
         assert L.cf_plan_supported(n, api.CF_PLAN_PLANE3) == (1 if n in (32, 64, 128) else 0)
-        for p in range(api.CF_PLAN_SPLIT3_G22, api.CF_PLAN_SPLIT3_G28M1 + 1):
+        four_step = list(range(api.CF_PLAN_SPLIT3_G22, api.CF_PLAN_SPLIT3_G28M1 + 1)) + \
+            [api.CF_PLAN_SPLIT3_G24M2O, api.CF_PLAN_SPLIT3_G28M2O]
+        for p in four_step:
             want = n in (128, 256)
             assert L.cf_plan_supported(n, p) == (1 if want else 0)
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5Z) == (1 if n >= 64 else 0)
-    assert api.CF_NUM_PLANS == api.CF_PLAN_AXIS5Z + 1
+    assert api.CF_NUM_PLANS == api.CF_PLAN_SPLIT3_G28M2O + 1 == len(api.PLAN_NAMES) + 1
     assert L.cf_plan_supported(1024, api.CF_PLAN_AXIS5) == 0
     assert L.cf_plan_supported(256, 99) == 0
     assert L.cf_set_plan(256, 99) == api.CF_ERR_ARG
@@ -194,10 +196,70 @@ def test_spectrum_close_waits_for_its_batches(monkeypatch):
             calls.append(("spectrum", h.value))
 
     monkeypatch.setattr(api, "lib", lambda: FakeLib())
-    monkeypatch.setattr(api, "_dev", lambda t: 0)
+    monkeypatch.setattr(api, "_dev", lambda t, *a, **k: 0)
     spec = api.Spectrum._wrap(ctypes.c_void_p(9), 8, None)
     b = api.Batch([(None, None, spec), (None, None, spec)])
     spec.close()
     assert calls == [] and spec.handle
     b.close()
     assert calls == [("batch", 55), ("spectrum", 9)] and not spec.handle
+
+
+class _FakeDeviceTensor:
+    """Stands in for a GPU tensor in the argument checks (no GPU here)."""
+
+    def __init__(self, numel, dtype="torch.complex64", esize=8, cuda=True, contiguous=True):
+        self._n, self.dtype, self._e, self.is_cuda, self._c = numel, dtype, esize, cuda, contiguous
+        self.device = "cuda:0" if cuda else "cpu"
+
+    def is_contiguous(self):
+        return self._c
+
+    def numel(self):
+        return self._n
+
+    def element_size(self):
+        return self._e
+
+    def data_ptr(self):
+        return 0x1000
+
+
+def test_device_buffers_are_checked_before_any_launch():
+    """The Python mirror refuses host tensors, non-contiguous or mistyped
+    tensors and tensors too small for the call, before the library runs a
+    kernel on them."""
+    import torch
+    from convfpga_amd import api
+    n = 8
+    spec = api.Spectrum._wrap(ctypes.c_void_p(0), n, None)  # never reaches the library
+    good = _FakeDeviceTensor(n ** 3)
+    cases = [
+        (torch.zeros(n ** 3, dtype=torch.complex64), TypeError),         # host memory
+        (_FakeDeviceTensor(n ** 3 - 1), ValueError),                     # one point short
+        (_FakeDeviceTensor(2 * n ** 3, "torch.float32", 4), TypeError),  # float32 cube
+        (_FakeDeviceTensor(n ** 3, contiguous=False), ValueError),
+        ("0x1000", TypeError),
+    ]
+    for bad, exc in cases:
+        with pytest.raises(exc):
+            api.conv3d_device(spec, bad, good)
+        with pytest.raises(exc):
+            api.conv3d_device(spec, good, bad)
+    with pytest.raises(ValueError):  # two cubes need twice the bytes
+        api.conv3d_device(spec, good, good, how_many=2)
+    with pytest.raises(ValueError):
+        api.fft3d_device(n, good, _FakeDeviceTensor(n ** 3 // 2))
+    with pytest.raises(ValueError):
+        api.Spectrum(n, _FakeDeviceTensor(n ** 3), count=2)
+    with pytest.raises(TypeError):  # an FP64 reference must be complex128
+        api.rel_l2_device(good, good, n ** 3, ref_is_f64=True)
+    with pytest.raises(ValueError):
+        api.fill_uniform(_FakeDeviceTensor(10), 21, 1)
+    with pytest.raises(ValueError):
+        api.fill_loguniform(_FakeDeviceTensor(10), 11, 1)
+    with pytest.raises(ValueError):
+        api.Batch([(good, _FakeDeviceTensor(3), spec)])
+    # float32 storage is fine for the fills, which count floats
+    assert api._dev(_FakeDeviceTensor(10, "torch.float32", 4), 40, "f32") == 0x1000
+    assert api._dev(12345, 1 << 30) == 12345  # a raw address is the caller's responsibility

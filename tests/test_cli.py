@@ -31,7 +31,7 @@ def test_help():
     rc, out = _run("--help")
     assert rc == 0
     for flag in ("-p, --path", "-n, --num", "-i, --iter", "-b, --batch", "-s, --usesvm", "-y, --noverify",
-                 "-c, --cpu-only", "-t, --threads", "-w, --wisdomfile", "-e, --emulate"):
+                 "-c, --cpu-only", "-t, --threads", "-w, --wisdomfile", "-e, --emulate", "--dump-out"):
         assert flag in out
 
 
@@ -66,6 +66,36 @@ def test_cpu_only_without_fftw():
     # src/main.cpp:28-31
     rc, out = _run("-c")
     assert rc == 1 and "FFTW not found" in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args,batch", [(("-n", "32"), 1), (("-s", "-n", "32"), 1), (("-s", "-b", "3", "-n", "32"), 3)])
+def test_cli_output_matches_oracle(args, batch, oracle, tmp_path):
+    """The numbers the harness computes, for the plain, the -s and the -s -b 3
+    dispatches (src/main.cpp:67-74): its inputs come from the reference's
+    generator (rand()/RAND_MAX in [0, 1], every instance a copy of the first,
+    src/helper.cpp:148-172; drawn after fpga_initialize as in main.cpp:42-62,
+    so where glibc's stream stands then is the runtime's business), and every
+    output cube equals the unnormalised convolution N^3 * (sig (*) filter) of
+    the FP64 oracle on exactly those inputs."""
+    import numpy as np
+    print("case convolution %s --dump-out" % " ".join(args), flush=True)
+    dump = str(tmp_path / "dump.bin")
+    rc, out = _run(*(args + ("-y", "--dump-out", dump)))
+    assert rc == 0, out
+    n = 32
+    P = n ** 3
+    data = np.fromfile(dump, dtype=np.complex64)
+    assert data.size == P + 2 * batch * P
+    filt, sig, got = data[:P], data[P:P + batch * P], data[P + batch * P:]
+    for x in (filt, sig):
+        f = x.view(np.float32)
+        assert f.min() >= 0.0 and f.max() <= 1.0 and 0.45 < f.mean() < 0.55
+    assert not np.array_equal(filt, sig[:P])  # two different draws
+    for b in range(batch):
+        assert np.array_equal(sig[b * P:(b + 1) * P], sig[:P])  # copies of the first (helper.cpp:166-171)
+        ref = oracle.conv3d_f64(n, np.ascontiguousarray(sig[b * P:(b + 1) * P]), filt)
+        assert oracle.rel_l2(np.ascontiguousarray(got[b * P:(b + 1) * P]), ref) <= 1e-5, b
 
 
 @pytest.mark.gpu

@@ -289,7 +289,8 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
-    [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)]
+    [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)] + \
+    [(n, p) for n in (128, 256) for p in (13, 14)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
@@ -331,7 +332,7 @@ def test_every_plan_matches_oracle(gpu, oracle, n, plan):
         api.set_plan(n, api.CF_PLAN_AUTO)
 
 
-@pytest.mark.parametrize("plan", [1, 3, 6, 12])
+@pytest.mark.parametrize("plan", [1, 3, 6, 12, 13])
 def test_dropin_256_on_each_plan(gpu, oracle, plan):
     n = 256
     print("case drop-in N=256 plan=%d" % plan, flush=True)
@@ -373,13 +374,13 @@ def _spectrum_order(api, n, plan, spec64):
         return spec64
     if name == "axis5z":
         return np.ascontiguousarray(spec64.reshape(n, n, n // 16, 16).transpose(2, 1, 0, 3)).reshape(-1)
-    tail = name.split("_g")[1]  # <GA><GB>[m<GM>]
+    tail = name.split("_g")[1].rstrip("o")  # <GA><GB>[m<GM>][o]
     gm = int(tail.split("m")[1]) if "m" in tail else int(tail[1])
     ref = spec64.reshape(n, n // 16, 16, n // gm, gm)  # (kz, k2, k1, kx/GM, kx%GM)
     return np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
 
 
-@pytest.mark.parametrize("n,plan", [(64, 1), (64, 2), (64, 12), (128, 9), (256, 4), (256, 7), (256, 8)])
+@pytest.mark.parametrize("n,plan", [(64, 1), (64, 2), (64, 12), (128, 9), (256, 4), (256, 7), (256, 8), (256, 14)])
 def test_spectrum_layout_of_each_plan(gpu, oracle, n, plan):
     """The filter spectrum a plan keeps (axis / plane passes: natural order;
     AXIS5Z: z-blocked; four-step: the convolution's first pass, then the
@@ -403,7 +404,7 @@ def test_spectrum_layout_of_each_plan(gpu, oracle, n, plan):
         assert oracle.rel_l2(got[k * P:(k + 1) * P], want) <= REL_L2_FP32, k
 
 
-@pytest.mark.parametrize("plan", [2, 5])
+@pytest.mark.parametrize("plan", [2, 5, 13])
 def test_svm_batch_on_3pass_plans(gpu, oracle, plan):
     n = 64 if plan == 2 else 256
     hm = 3 if n == 64 else 2
@@ -641,6 +642,27 @@ def test_batch_graph_keeps_order_of_dependent_instances(gpu, oracle):
     same.check()
     assert oracle.rel_l2(_np(fin), oracle.conv3d_f64(n, s2, f2, 1.0 / P)) <= REL_L2_FP32
     same.close()
+
+
+def test_persistent_batch_refuses_overlapping_instances(gpu, oracle):
+    """Instances of a persistent batch run concurrently: an output that
+    overlaps another instance's signal or output, or part of its own signal,
+    is refused (graph mode keeps the order and accepts a chain)."""
+    import torch
+    api = gpu.api
+    n = 32
+    P = n ** 3
+    print("case persistent batch overlap check", flush=True)
+    spec = gpu.Spectrum(n, _t(oracle.uniform(P, 73001)))
+    buf = torch.zeros(3 * P, dtype=torch.complex64, device="cuda")
+    a, b, c = buf[:P], buf[P:2 * P], buf[2 * P:]
+    for inst in ([(a, b, spec), (b, c, spec)],        # chain: 2 reads 1's output
+                 [(a, c, spec), (b, c, spec)],        # one output for two instances
+                 [(a, buf[P // 2:P // 2 + P], spec)]):  # output half over its own signal
+        with pytest.raises(gpu.ConvError, match="overlap"):
+            api.Batch(inst, mode=api.CF_BATCH_PERSISTENT)
+    api.Batch([(a, b, spec), (b, c, spec)], mode=api.CF_BATCH_GRAPH).close()
+    api.Batch([(a, a, spec), (b, b, spec)], mode=api.CF_BATCH_PERSISTENT).close()  # in place: fine
 
 
 def test_batch_shared_spectrum_in_place(gpu, oracle):

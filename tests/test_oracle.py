@@ -178,3 +178,109 @@ def test_uniform_range_and_determinism(oracle):
     assert not np.array_equal(a, oracle.uniform(1 << 16, 43))
     m = np.abs(oracle.loguniform(1 << 14, 1))
     assert 9e-4 < m.min() and m.max() < 1.1e3
+
+
+# ---- the FP32 restatement and the metrics, each pinned on its own ----------
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_fft_f32_close_to_f64_and_numpy(oracle, n):
+    """orc_fft3d_f32 against the pinned FP64 transform and numpy, both
+    directions: FP32 rounding only (a wrong sign, index or dropped stage is
+    an O(1) error; a silent FP64 path would be below 1e-9)."""
+    x = oracle.uniform(n ** 3, 300 + n)
+    for sign in (-1, +1):
+        a = oracle.fft3d_f32(x, sign)
+        b = oracle.fft3d_f64(x, sign)
+        err = oracle.rel_l2(a, b)
+        assert 1e-9 < err < 1e-6, (sign, err)
+    npref = np.fft.fftn(x.astype(np.complex128).reshape(n, n, n)).ravel()
+    assert np.linalg.norm(oracle.fft3d_f32(x, -1) - npref) / np.linalg.norm(npref) < 1e-6
+
+
+def test_fft_f32_single_tone_sign_and_amplitude(oracle):
+    """A tone exp(+2 pi i k.r / N) lands in bin k with amplitude P = N^3 under
+    the forward sign -; the backward transform (sign +) puts it in bin -k."""
+    n = 16
+    kx, ky, kz = 5, 2, 9
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    tone = np.exp(2j * np.pi * (kx * x + ky * y + kz * z) / n).ravel().astype(np.complex64)
+    spec = oracle.fft3d_f32(tone, -1)
+    peak = (kz * n + ky) * n + kx
+    assert abs(spec[peak] - n ** 3) < 1e-2
+    spec[peak] = 0
+    assert np.abs(spec).max() < 1e-2
+    back = oracle.fft3d_f32(tone, +1)
+    mirror = (((n - kz) % n) * n + (n - ky) % n) * n + (n - kx) % n
+    assert abs(back[mirror] - n ** 3) < 1e-2
+    back[mirror] = 0
+    assert np.abs(back).max() < 1e-2
+
+
+def test_fft_f32_round_trip_and_thread_count(oracle):
+    """IFFT(FFT(x)) = P x; every line is transformed by itself, so the result
+    does not depend on the number of threads (bit-equal)."""
+    n = 32
+    x = oracle.uniform(n ** 3, 77)
+    X1 = oracle.fft3d_f32(x, -1, nthreads=1)
+    X4 = oracle.fft3d_f32(x, -1, nthreads=4)
+    assert np.array_equal(X1.view(np.uint64), X4.view(np.uint64))
+    back = oracle.fft3d_f32(X1, +1) / np.float32(n ** 3)
+    assert oracle.rel_l2(back.astype(np.complex64), x) < 1e-6
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_conv_f32_spec_equals_conv_f32(oracle, n):
+    """The timed CPU baseline step (spectrum precomputed) performs exactly the
+    operations of the FP32 restatement of the reference verifier: bit-equal
+    output with H = fft3d_f32(filter), for one thread and for several."""
+    sig, filt = oracle.uniform(n ** 3, 500 + n), oracle.uniform(n ** 3, 600 + n)
+    H = oracle.fft3d_f32(filt, -1)
+    want = oracle.conv3d_f32(n, sig, filt)
+    for threads in (1, 3):
+        out = np.empty_like(sig)
+        oracle.conv3d_f32_spec(n, sig, H, out, nthreads=threads)
+        assert np.array_equal(out.view(np.uint64), want.view(np.uint64))
+    # and it is the convolution (pinned FP64 oracle)
+    assert oracle.rel_l2(want, oracle.conv3d_f64(n, sig, filt)) < 1e-6
+
+
+def test_rel_l2_matches_numpy(oracle):
+    a = oracle.uniform(4096, 1)
+    b = (a + np.complex64(3e-3) * oracle.uniform(4096, 2)).astype(np.complex64)
+    a64, b64 = a.astype(np.complex128), b.astype(np.complex128)
+    want = np.linalg.norm(b64 - a64) / np.linalg.norm(a64)
+    assert abs(oracle.rel_l2(b, a) - want) <= 1e-12 * want  # f32 vs f32 reference
+    ref64 = a64 + 1e-9j  # an FP64 reference that float32 cannot hold
+    want64 = np.linalg.norm(b64 - ref64) / np.linalg.norm(ref64)
+    assert abs(oracle.rel_l2(b, ref64) - want64) <= 1e-12 * want64
+    # zero reference: the absolute norm; identical arrays: zero
+    z = np.zeros(16, np.complex64)
+    assert abs(oracle.rel_l2(b[:16].copy(), z) - np.linalg.norm(b64[:16])) < 1e-9
+    assert oracle.rel_l2(a, a) == 0.0 and oracle.rel_l2(a, a64) == 0.0
+
+
+def test_fftw_probe_and_threads(oracle):
+    """The FFTW probe agrees with a direct dlopen of the library; the thread
+    count is the OpenMP default (at least one)."""
+    try:
+        L = ctypes.CDLL("libfftw3f.so.3")
+        have = all(hasattr(L, s) for s in ("fftwf_plan_many_dft", "fftwf_execute", "fftwf_destroy_plan"))
+    except OSError:
+        have = False
+    assert oracle.fftw_available() == have
+    assert oracle.max_threads() >= 1
+
+
+def test_loguniform_matches_numpy_port(oracle):
+    """orc_fill_loguniform against the numpy restatement of the generator
+    (tests/golden/make_golden.py), value by value to one float ulp (libm's
+    pow/cos/sin and numpy's may differ in the last bit of a double)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(os.path.dirname(__file__), "golden",
+                                                                     "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    got = oracle.loguniform(1 << 12, 9)
+    want = mg.loguniform(1 << 12, 9)
+    assert np.allclose(got.view(np.float32), want.view(np.float32), rtol=2 ** -23, atol=0)
+    assert np.array_equal(oracle.uniform(1 << 12, 9).view(np.uint32), mg.uniform(1 << 12, 9).view(np.uint32))
