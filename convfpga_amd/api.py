@@ -25,6 +25,7 @@ CF_ERR_ARG, CF_ERR_SIZE, CF_ERR_STATE, CF_ERR_RUNTIME, CF_ERR_NOMEM = -1, -2, -3
 CF_SCALE_NONE, CF_SCALE_INV_N3 = 0, 1
 CF_PREC_FP32, CF_PREC_SPLIT_FP16 = 0, 1
 CF_MAX_PASSES = 8
+CF_NOALLOC_SCRATCH_BYTES = 128 << 20
 CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
 CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
 CF_PLAN_SPLIT3_G24M2, CF_PLAN_SPLIT3_G24M1, CF_PLAN_SPLIT3_G28M4, CF_PLAN_SPLIT3_G28M2, CF_PLAN_SPLIT3_G28M1 = 7, 8, 9, 10, 11
@@ -41,7 +42,7 @@ SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 DROPIN_SYMBOLS = ("fpga_initialize", "fpga_final", "fpgaf_complex_malloc", "fpgaf_conv3D",
                   "fpgaf_conv3D_svm", "fpgaf_conv3D_svm_batch")
 EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_spectrum_destroy",
-               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device", "cf_conv3d_host",
+               "cf_spectrum_data", "cf_spectrum_edge", "cf_spectrum_count", "cf_spectrum_view", "cf_conv3d_device", "cf_conv3d_device_noalloc", "cf_conv3d_host",
                "cf_conv3d_device_timed", "cf_conv3d_share_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
@@ -108,6 +109,7 @@ def lib():
             "cf_spectrum_count": (u, [vp]),
             "cf_spectrum_view": (i, [vp, u, ctypes.POINTER(vp)]),
             "cf_conv3d_device": (i, [vp, vp, vp, u, i, i, vp]),
+            "cf_conv3d_device_noalloc": (i, [vp, vp, vp, u, i, i, vp]),
             "cf_conv3d_host": (i, [vp, vp, vp, u, i, i, ctypes.POINTER(cf_host_times_t)]),
             "cf_conv3d_device_timed": (i, [vp, vp, vp, u, i, i, vp, ctypes.POINTER(cf_pass_times_t)]),
             "cf_conv3d_share_timed": (i, [vp, vp, vp, u, i, vp, ctypes.POINTER(cf_pass_times_t)]),
@@ -322,6 +324,15 @@ def conv3d_device(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=C
     _check(lib().cf_conv3d_device(spec.handle, _dev(sig, nb), _dev(out, nb), how_many, scale, precision,
                                   _stream(stream)),
            "cf_conv3d_device")
+
+
+def conv3d_device_noalloc(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32, stream=None):
+    """cf_conv3d_device without any allocation: the intermediate of the
+    four-step and z-blocked plans lives in the code object's fixed scratch
+    (CF_NOALLOC_SCRATCH_BYTES); calls overlapping on the GPU race."""
+    nb = 8 * spec.N ** 3 * how_many
+    _check(lib().cf_conv3d_device_noalloc(spec.handle, _dev(sig, nb), _dev(out, nb), how_many, scale, precision,
+                                          _stream(stream)), "cf_conv3d_device_noalloc")
 
 
 def conv3d_host(spec, sig, out, how_many=1, scale=CF_SCALE_INV_N3, precision=CF_PREC_FP32):

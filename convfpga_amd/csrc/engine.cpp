@@ -139,6 +139,10 @@ struct Engine {
   void* d_err = nullptr;
   size_t d_err_bytes = 0;
   hipEvent_t ev[CF_MAX_PASSES + 8] = {};
+  // device address of the code object's cf_scratch array on g.device (the
+  // only device this engine drives), looked up once when the module is loaded
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
   // cf_profile_begin/end: events of up to prof_max calls, (CF_MAX_PASSES+1) each
   std::vector<hipEvent_t> prof_ev;
   unsigned prof_max = 0, prof_calls = 0, prof_npass = 0;
@@ -203,6 +207,8 @@ void release_all() {
   if (g.mod) (void)hipModuleUnload(g.mod);
   g.mod = nullptr;
   g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_reset = nullptr;
+  g.scratch = nullptr;  // part of the unloaded module
+  g.scratch_bytes = 0;
   if (g.capture) (void)hipStreamDestroy(g.capture);
   g.capture = nullptr;
   g.ready = false;
@@ -1100,6 +1106,22 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
     release_all();
     return -4;
   }
+  // The allocation-free entry point's scratch: a device array of the module,
+  // allocated by the module load.  Its address on this device comes from the
+  // runtime (the host never names the array itself).
+  {
+    hipDeviceptr_t p = nullptr;
+    size_t bytes = 0;
+    const hipError_t e = hipModuleGetGlobal(&p, &bytes, g.mod, "cf_scratch");
+    if (e != hipSuccess || !p || bytes != (size_t)cfgeo::noalloc_scratch_bytes) {
+      fprintf(stderr, "convfpga: '%s' has no scratch array of %llu bytes (%s)\n", path,
+              (unsigned long long)cfgeo::noalloc_scratch_bytes, hipGetErrorString(e));
+      release_all();
+      return -4;
+    }
+    g.scratch = (void*)p;
+    g.scratch_bytes = bytes;
+  }
   g.ready = true;
   return 0;
 }
@@ -1417,6 +1439,39 @@ extern "C" int cf_conv3d_device(cf_spectrum_t h, const void* d_sig, void* d_out,
                                 int precision, void* stream) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   return conv_device_impl(h, d_sig, d_out, how_many, scale_mode, precision, stream, nullptr);
+}
+
+static_assert(CF_NOALLOC_SCRATCH_BYTES == cfgeo::noalloc_scratch_bytes, "scratch size of the header and the kernels");
+
+extern "C" int cf_conv3d_device_noalloc(cf_spectrum_t h, const void* d_sig, void* d_out, unsigned how_many,
+                                        int scale_mode, int precision, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g.mu);
+  if (!h || !d_sig || !d_out || how_many == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
+  CFCHK(check_conv_args(h, how_many, scale_mode, precision));
+  CFCHK(require_ready());
+  // the edge's kernels and twiddles were set up when the spectrum was made;
+  // nothing is created here
+  auto it = g.sizes.find(h->n);
+  if (it == g.sizes.end())
+    return fail(CF_ERR_STATE, "N = %u has not been set up in this session: create its spectrum after fpga_initialize()",
+                h->n);
+  const SizeFns& f = it->second;
+  const unsigned n = h->n;
+  void* work = nullptr;
+  if (needs_work(h->plan)) {
+    const size_t need = work_bytes(h->plan, n, how_many, d_sig, d_out);
+    if (need > g.scratch_bytes)
+      return fail(CF_ERR_SIZE,
+                  "plan %d of N = %u needs a %zu-byte intermediate for %u cubes; the allocation-free call has %zu "
+                  "bytes of scratch (fewer cubes per call, or cf_conv3d_device)",
+                  h->plan, n, need, how_many, g.scratch_bytes);
+    work = g.scratch;
+  }
+  const float scale = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+  const long long hstride = h->count == 1 ? 0 : (long long)cube_elems(n);
+  PassList p;
+  conv_passes(f, h->plan, n, how_many, d_sig, d_out, work, h->d, hstride, scale, p, precision);
+  return run_passes(p, n, how_many, (hipStream_t)stream, f.tw, nullptr);
 }
 
 extern "C" int cf_conv3d_host(cf_spectrum_t h, const void* h_sig, void* h_out, unsigned how_many, int scale_mode,

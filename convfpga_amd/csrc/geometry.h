@@ -57,4 +57,8 @@ constexpr int split_a_tiles(int n, int v) { return n * ((n / 16) / split_ga(v));
 constexpr int split_b_threads(int n, int v) { return (n / 16) * split_gm(v) * (n / 16); }
 constexpr int split_b_tiles(int n, int v) { return 16 * (n / split_gm(v)); }
 
+// device scratch of the allocation-free entry point (cf_scratch in the code
+// object): one 256^3 cube, the four-step intermediate of one convolution
+constexpr unsigned long long noalloc_scratch_bytes = 128ull << 20;
+
 }  // namespace cfgeo

@@ -320,3 +320,14 @@ extern "C" __global__ __launch_bounds__(256) void cf_fill_loguniform(float2* out
 }
 
 
+// ---- scratch of the allocation-free entry point ----------------------------------
+// cf_conv3d_device_noalloc runs the four-step and z-blocked plans with their
+// intermediate cube in this array instead of a workspace it would have to
+// allocate: 128 MiB, the intermediate of one 256^3 convolution (or 8 at 128^3,
+// 64 at 64^3).  It is part of the code object's zero-initialised data, so it
+// is allocated once when fpga_initialize() loads the code object and held
+// until fpga_final() unloads it.  The host takes its device address from the
+// loaded module by name (hipModuleGetGlobal), hence external linkage.
+extern "C" {
+__device__ __attribute__((used)) __align__(256) float2 cf_scratch[cfgeo::noalloc_scratch_bytes / sizeof(float2)];
+}

@@ -163,6 +163,25 @@ int cf_conv3d_share_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsig
 int cf_profile_begin(unsigned max_calls);
 int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
 
+/* ---- the same without any allocation ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments and result as cf_conv3d_device, but the
+ * intermediate cube of the four-step and z-blocked plans lives in a fixed
+ * device scratch array of the code object (allocated when fpga_initialize()
+ * loads it, held until fpga_final()), so the call allocates and frees
+ * nothing: it only launches the passes on `stream`.  Limit: an intermediate
+ * of at most CF_NOALLOC_SCRATCH_BYTES (N^3 * 8 * how_many bytes: one 256^3
+ * cube, 8 at 128^3, 64 at 64^3) -- larger calls fail with CF_ERR_SIZE; plans
+ * without an intermediate (CF_PLAN_AXIS5, CF_PLAN_PLANE3) have no limit.
+ * Every call shares the one scratch array: calls that overlap in time on the
+ * GPU (on different streams, or from different threads) race.  The spectrum
+ * must have been created in the current session (its edge's kernels and
+ * twiddles are set up then). */
+#define CF_NOALLOC_SCRATCH_BYTES 134217728ull
+int cf_conv3d_device_noalloc(cf_spectrum_t h, const void *d_sig, void *d_out, unsigned how_many, int scale_mode,
+                             int precision, void *stream);
+
 /* ---- batches of mixed sizes ------------------------------------------------
  * A fixed list of convolutions (each: device signal, device output, spectrum
  * of count 1 -- instances may share a spectrum) prepared once and replayed
