@@ -10,8 +10,10 @@ scale.  Inputs are synthetic (seeded U[-1,1) re/im, generated in HBM by the
 library).  Prints one JSON line.
 
 Per-pass device times come from HIP events recorded by the library on the
-launch stream during the timed steps; the roofline entry is for the pass with
-the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
+launch stream during the timed loop, on one step in PROFILE_EVERY (each
+event packet between two passes costs the GPU a few microseconds, so
+instrumenting every step would slow the loop it measures); the roofline
+entry is for the pass with the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
 convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
 against the FP64 oracle.  Beside the FP32 form the line carries the same
@@ -40,6 +42,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
+PROFILE_EVERY = 4  # steps of the timed loop that carry per-pass events: one in PROFILE_EVERY
 
 SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64, "c5_512": 512}
 SPLIT = {"c5_split": [256] * 4 + [512]}
@@ -168,19 +171,26 @@ def base_line(a, metric, value, unit, ms_per_step, config):
             "dtype": "fp32", "data": "synthetic (seeded U[-1,1) re/im generated on the GPU)", "config": config}
 
 
-KERNEL_OF_PASS = {  # kernel of each pass name (N = edge; G, GM = tile-group suffixes of the four-step plans)
-    "xy1_fwd": "cf_xy1_fwd_{N}{G}", "xy1_inv": "cf_xy1_inv_{N}{G}", "y2z_fwd_mul_inv": "cf_mid_{N}{GM}",
+KERNEL_OF_PASS = {  # kernel of each pass name of the axis and plane schedules (N = edge)
     "x_fwd": "cf_row_fwd_{N}", "x_inv": "cf_row_inv_{N}", "y_fwd": "cf_coly_fwd_{N}", "y_inv": "cf_coly_inv_{N}",
     "z_fwd_mul_z_inv": "cf_colz_conv_{N}", "xy_fwd": "cf_plane_fwd_{N}", "xy_inv": "cf_plane_inv_{N}",
 }
 
 
 def kernel_of_pass(name, n, plan_name):
-    gm = "_" + plan_name.split("_")[1] if plan_name.startswith("split3_") else ""
-    g = gm.split("m")[0]
-    if name == "xy1_inv" and gm.endswith("o"):  # reads the middle pass's second intermediate: x group GM
-        g = g[:-1] + gm[-2]
-    k = KERNEL_OF_PASS.get(name, "").format(N=n, G=g, GM=gm)
+    """Kernel of pass `name` under a plan: four-step plans split3_g<GA><GB>[m<GM>][o|h]
+    (o: second intermediate of x group GM; h: y2 pairs interleaved)."""
+    if plan_name.startswith("split3_"):
+        v = plan_name.split("_")[1]
+        g = v.split("m")[0]
+        gi = g
+        if v.endswith("o"):  # the last pass reads x groups of GM
+            gi = g[:-1] + v[-2]
+        elif v.endswith("h"):
+            gi = g + "h"
+        return {"xy1_fwd": "cf_xy1_fwd_%d_%s" % (n, g), "xy1_inv": "cf_xy1_inv_%d_%s" % (n, gi),
+                "y2z_fwd_mul_inv": "cf_mid_%d_%s" % (n, v)}.get(name, "")
+    k = KERNEL_OF_PASS.get(name, "").format(N=n)
     if plan_name == "axis5z" and name in ("y_fwd", "y_inv", "z_fwd_mul_z_inv"):
         k = k.replace("_%d" % n, "_zb_%d" % n)  # the z-blocked column kernels
     return k
@@ -210,17 +220,20 @@ def run_single(a, torch, C, api):
     for _ in range(a.warmup):
         C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
-    api.profile_begin(a.steps)
+    # per-pass HIP events on every PROFILE_EVERY-th step of the timed loop
+    # (an event packet between two passes costs the GPU a few microseconds)
+    every = PROFILE_EVERY if a.steps >= 2 * PROFILE_EVERY else 1
+    recorded = (a.steps + every - 1) // every
+    api.profile_begin(recorded, every)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     passes, ncalls = api.profile_end()
-    assert ncalls == a.steps
+    assert ncalls == recorded
     wall = t1 - t0
-    # the same loop without the per-pass events (each event packet is a short
-    # gap on the GPU: a few percent at 256^3, more for launch-bound small cubes)
+    # the same loop without any per-pass events
     t2 = time.perf_counter()
     for _ in range(a.steps):
         C.conv3d_device(spec, sig, out)
@@ -250,6 +263,7 @@ def run_single(a, torch, C, api):
     line["conv_roofline_frac_5pass_model"] = 11 * 8.0 * P * a.steps / wall / 1e9 / HBM_PEAK_GBS
     line["device_ms_per_step"] = sum(p["ms"] for p in passes)
     line["conv_per_s_without_pass_events"] = a.steps / wall_plain
+    line["steps_with_pass_events"] = recorded
     # for information: the compulsory floor (read signal and spectrum, write the
     # output: 3V at the HBM peak) and the arithmetic (two 3D FFTs, the product)
     floor_us = 3 * 8.0 * P / (HBM_PEAK_GBS * 1e9) * 1e6
@@ -264,6 +278,8 @@ def run_single(a, torch, C, api):
     host_out = out.cpu().numpy() if not a.no_cpu_baseline else None
     scratch = torch.empty_like(out)  # the legs below write their own outputs here
     line["own_filter"] = own_filter_leg(torch, C, n, sig, filt, scratch)
+    if n <= 128:  # small cubes: all passes in one launch
+        line["one_launch_persistent"] = one_launch_leg(a, torch, C, api, n, sig, filt, scratch, 7 * 8.0 * P)
     line["slab_share"] = share_leg(C, n, spec, sig, scratch)
     split_host = None
     if n == 256:
@@ -419,6 +435,45 @@ def host_batch_leg(torch, C, n, spec, count, reps=3):
             "conv_per_s": 1e3 * count / w, "host_bus_GBps_both_ways": 2 * 8.0 * P * count / (w * 1e-3) / 1e9}
 
 
+def persistent_plan(api, n):
+    """Plan of the spectra of persistent-mode batches: the plane plan at 32^3
+    and 64^3, the four-step plan with interleaved y2 pairs at 128^3 and 256^3
+    (the plans the persistent kernel runs), the axis plan otherwise."""
+    return {32: api.CF_PLAN_PLANE3, 64: api.CF_PLAN_PLANE3, 128: api.CF_PLAN_SPLIT3_G24M2H,
+            256: api.CF_PLAN_SPLIT3_G24M2H}.get(n, api.CF_PLAN_AXIS5)
+
+
+def one_launch_leg(a, torch, C, api, n, sig, filt, out, alg_bytes):
+    """The same convolution as one launch of the persistent kernel (a batch of
+    one instance, all three passes in one kernel, stages ordered on the
+    device), timed like the main loop; its output is checked against the main
+    loop's (bitwise: the same passes)."""
+    api.set_plan(n, persistent_plan(api, n))
+    try:
+        sp = C.Spectrum(n, filt)
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    o1 = torch.empty_like(out)
+    b = api.Batch([(sig, o1, sp)], mode=api.CF_BATCH_PERSISTENT)
+    for _ in range(a.warmup):
+        b.launch()
+    b.check()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b.launch()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    b.check()
+    o2 = torch.empty_like(out)
+    C.conv3d_device(sp, sig, o2)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(o1.view(torch.int64), o2.view(torch.int64)))
+    b.close()
+    return {"plan": api.PLAN_NAMES[sp.plan], "ms_per_conv": 1e3 * t, "conv_per_s": 1.0 / t,
+            "roofline_frac_7V": alg_bytes / t / 1e9 / HBM_PEAK_GBS, "bitwise_equal_to_three_launches": same}
+
+
 def run_batch(a, torch, C, api):
     sizes = list(BATCHES[a.workload])
     shared = a.workload == "c3_64x64"
@@ -460,16 +515,13 @@ def run_batch(a, torch, C, api):
             res.append((sig[n][k * P:(k + 1) * P], out[n][k * P:(k + 1) * P], specs[n][0 if shared else k]))
         return res
 
-    # graph mode: spectra of each edge's autotuned plan; persistent mode:
-    # spectra in natural order (its layout), made under the axis plan for every
-    # edge (the autotuned plan of any edge from 64 up, four-step or z-blocked,
-    # may lay its spectrum out privately)
+    # graph mode: spectra of each edge's autotuned plan; persistent mode: the
+    # plans its kernel runs (PERSISTENT_PLANS)
     inst_g = instances(make_specs())
-    natural = list(by_n)
-    for n in natural:
-        api.set_plan(n, api.CF_PLAN_AXIS5)
+    for n in by_n:
+        api.set_plan(n, persistent_plan(api, n))
     inst = instances(make_specs())
-    for n in natural:
+    for n in by_n:
         api.set_plan(n, api.CF_PLAN_AUTO)
     # graph mode records contiguous runs of one spectrum as batched passes: order by edge
     inst_sorted = sorted(inst_g, key=lambda t: (t[2].N, t[0].data_ptr()))
@@ -520,12 +572,12 @@ def run_batch(a, torch, C, api):
     wall = results[best]
 
     def npass(plan):
-        return 5 if plan == api.CF_PLAN_AXIS5 else 3
+        return 5 if plan in (api.CF_PLAN_AXIS5, api.CF_PLAN_AXIS5Z) else 3
 
     # algorithmic bytes of the executed schedule: 2 volumes per pass per cube,
     # plus one spectrum read per distinct spectrum
     if best.startswith("persistent"):
-        passes_of = {id(h): 5 for _, _, h in inst}
+        passes_of = {id(h): npass(h.plan) for _, _, h in inst}
     elif best.startswith("batched"):
         passes_of = {id(h): npass(sp.plan) for _, _, h in inst}
     else:

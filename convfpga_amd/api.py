@@ -30,12 +30,13 @@ CF_PLAN_AUTO, CF_PLAN_AXIS5, CF_PLAN_PLANE3 = 0, 1, 2
 CF_PLAN_SPLIT3_G22, CF_PLAN_SPLIT3_G24, CF_PLAN_SPLIT3_G42, CF_PLAN_SPLIT3_G44 = 3, 4, 5, 6
 CF_PLAN_SPLIT3_G24M2, CF_PLAN_SPLIT3_G24M1, CF_PLAN_SPLIT3_G28M4, CF_PLAN_SPLIT3_G28M2, CF_PLAN_SPLIT3_G28M1 = 7, 8, 9, 10, 11
 CF_PLAN_AXIS5Z = 12
-CF_PLAN_SPLIT3_G24M2O, CF_PLAN_SPLIT3_G28M2O = 13, 14
-CF_NUM_PLANS = 15
+CF_PLAN_SPLIT3_G24M2O, CF_PLAN_SPLIT3_G28M2O, CF_PLAN_SPLIT3_G24M2H, CF_PLAN_SPLIT3_G28M2H = 13, 14, 15, 16
+CF_NUM_PLANS = 17
 CF_BATCH_PERSISTENT, CF_BATCH_GRAPH = 0, 1
 PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44",
               7: "split3_g24m2", 8: "split3_g24m1", 9: "split3_g28m4", 10: "split3_g28m2", 11: "split3_g28m1",
-              12: "axis5z", 13: "split3_g24m2o", 14: "split3_g28m2o"}
+              12: "axis5z", 13: "split3_g24m2o", 14: "split3_g28m2o",
+              15: "split3_g24m2h", 16: "split3_g28m2h"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
 # names of every symbol include/convfpga/*.h declares
@@ -46,7 +47,7 @@ EXT_SYMBOLS = ("cf_last_error", "cf_supported_size", "cf_spectrum_create", "cf_s
                "cf_conv3d_device_timed", "cf_conv3d_share_timed", "cf_fft3d_device", "cf_rel_l2_device", "cf_profile_begin",
                "cf_profile_end", "cf_fill_uniform", "cf_fill_loguniform", "cf_plan_supported", "cf_set_plan", "cf_get_plan",
                "cf_autotune", "cf_spectrum_plan", "cf_batch_create", "cf_batch_launch", "cf_batch_check",
-               "cf_batch_destroy")
+               "cf_batch_destroy", "cf_batch_create_stall_test")
 
 
 class fpga_t(ctypes.Structure):
@@ -115,7 +116,7 @@ def lib():
             "cf_conv3d_share_timed": (i, [vp, vp, vp, u, i, vp, ctypes.POINTER(cf_pass_times_t)]),
             "cf_fft3d_device": (i, [u, vp, vp, u, i, vp]),
             "cf_rel_l2_device": (i, [vp, vp, i, sz, vp, ctypes.POINTER(d)]),
-            "cf_profile_begin": (i, [u]),
+            "cf_profile_begin": (i, [u, u]),
             "cf_fill_uniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
             "cf_fill_loguniform": (i, [vp, sz, ctypes.c_ulonglong, vp]),
             "cf_plan_supported": (i, [u, i]),
@@ -127,6 +128,8 @@ def lib():
             "cf_batch_launch": (i, [vp, vp]),
             "cf_batch_check": (i, [vp]),
             "cf_batch_destroy": (None, [vp]),
+            "cf_batch_create_stall_test": (i, [ctypes.POINTER(cf_instance_t), u, i, ctypes.c_ulonglong,
+                                               ctypes.POINTER(vp)]),
             "cf_profile_end": (i, [ctypes.POINTER(cf_pass_times_t), ctypes.POINTER(u)]),
         }
         for name, (res, args) in sig.items():
@@ -378,8 +381,10 @@ def rel_l2_device(test, ref, npts, ref_is_f64, stream=None):
     return r.value
 
 
-def profile_begin(max_calls):
-    _check(lib().cf_profile_begin(max_calls), "cf_profile_begin")
+def profile_begin(max_calls, every=1):
+    """Per-pass events on every `every`-th of the next cf_conv3d_device calls,
+    up to max_calls recorded calls."""
+    _check(lib().cf_profile_begin(max_calls, every), "cf_profile_begin")
 
 
 def profile_end():
@@ -428,14 +433,21 @@ class Batch:
     graph launch (cf_batch_t).  Keeps its signals, outputs and spectra alive
     until it is closed (a spectrum closed meanwhile is destroyed then)."""
 
-    def __init__(self, instances, scale=CF_SCALE_INV_N3, mode=CF_BATCH_PERSISTENT):
+    def __init__(self, instances, scale=CF_SCALE_INV_N3, mode=CF_BATCH_PERSISTENT, stall_test_spin_limit=None):
+        """stall_test_spin_limit: build the test-only batch of
+        cf_batch_create_stall_test instead (persistent mode; its last stage
+        waits for a tile that never comes, at most that many polls)."""
         self._h, self._keep, self._specs, self.mode = ctypes.c_void_p(), list(instances), [], mode
         arr = (cf_instance_t * len(instances))()
         for k, (sig, out, spec) in enumerate(instances):
             nb = 8 * spec.N ** 3
             arr[k] = cf_instance_t(_dev(sig, nb), _dev(out, nb), spec.handle)
         h = ctypes.c_void_p()
-        _check(lib().cf_batch_create(arr, len(instances), scale, mode, ctypes.byref(h)), "cf_batch_create")
+        if stall_test_spin_limit is None:
+            _check(lib().cf_batch_create(arr, len(instances), scale, mode, ctypes.byref(h)), "cf_batch_create")
+        else:
+            _check(lib().cf_batch_create_stall_test(arr, len(instances), scale, stall_test_spin_limit,
+                                                    ctypes.byref(h)), "cf_batch_create_stall_test")
         self._h = h
         for spec in {id(i[2]): i[2] for i in instances}.values():
             spec._acquire()

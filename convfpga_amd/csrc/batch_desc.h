@@ -3,21 +3,43 @@
 // Plain types only; 64 bytes.
 #pragma once
 
+// One segment = all tiles of one pass of one instance.  Segments are ordered
+// stage-major (stage = the pass's index in its instance's schedule), so the
+// work queue hands out every tile of stage s before any tile of stage s + 1.
 struct CfBatchSeg {
   const void* in;   // pass input (complex float cube)
   void* out;        // pass output
-  const void* H;    // filter spectrum (z*H*z pass only)
+  const void* H;    // filter spectrum (the pass that multiplies by it)
   const void* tw;   // FP32 twiddle table W_n^j of edge n
-  float scale;      // final scale (x^-1 pass)
+  float scale;      // final scale (last inverse pass)
   int n;            // cube edge
-  int kind;         // 0 x fwd, 1 y fwd, 2 z*H*z, 3 y inv, 4 x inv
+  int kind;         // CfBatchKind
+  int variant;      // four-step tile variant (cfgeo::split_*), 0 otherwise
   int first_item;   // queue index of the segment's first work item
   int ntiles;       // tiles of the pass
-  int per_item;     // tiles per work item (the last item of a segment may hold fewer)
-  int dep;          // index of the segment that must complete first, -1 if none
-  int dep_need;     // tile count of that segment
+  int per_item;     // tiles per work item (a multiple of the tiles a work-group runs at once)
+  int stage;        // stage of the pass
 };
 static_assert(sizeof(CfBatchSeg) == 64, "CfBatchSeg must be 64 bytes");
 
-constexpr int kBatchKinds = 5;
-constexpr int kBatchThreads = 256;
+enum CfBatchKind : int {
+  CF_BK_ROW_FWD = 0,    // x pass (five-pass schedule)
+  CF_BK_COLY_FWD = 1,   // y pass
+  CF_BK_COLZ_CONV = 2,  // z, product with the spectrum, z^-1 (five-pass and plane schedules)
+  CF_BK_COLY_INV = 3,   // y^-1
+  CF_BK_ROW_INV = 4,    // x^-1 and scale
+  CF_BK_PLANE_FWD = 5,  // xy plane (plane schedule, N = 32, 64)
+  CF_BK_PLANE_INV = 6,  // xy plane^-1 and scale
+  CF_BK_XY1_FWD = 7,    // four-step outer pass (N = 128, 256)
+  CF_BK_MID = 8,        // four-step middle pass with the product
+  CF_BK_XY1_INV = 9,    // four-step outer pass^-1 and scale
+};
+
+constexpr int kBatchThreads = 512;  // threads of a work-group of the persistent kernel
+constexpr int kBatchMaxStages = 5;  // passes of the longest schedule
+// counters: [0] queue head, [1] error flag, [2] work-groups finished,
+// [3 + s] tiles of stage s completed; each on its own 256-byte line
+// (counter c at unsigned index c * kBatchCtrStride), so that the polls of
+// waiting work-groups do not contend with the queue's atomics
+constexpr int kBatchCounters = 3 + kBatchMaxStages;
+constexpr int kBatchCtrStride = 64;

@@ -78,14 +78,14 @@ struct SizeFns {
 // four-step plans: CF_PLAN_SPLIT3_G22 .. G28M1 are variants 0..8, then
 // CF_PLAN_SPLIT3_G24M2O and G28M2O variants 9 and 10 (geometry.h)
 bool is_split(int plan) {
-  return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) || plan == CF_PLAN_SPLIT3_G24M2O ||
-         plan == CF_PLAN_SPLIT3_G28M2O;
+  return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) ||
+         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2H);
 }
 int split_variant(int plan) {
   return plan <= CF_PLAN_SPLIT3_G28M1 ? plan - CF_PLAN_SPLIT3_G22 : 9 + (plan - CF_PLAN_SPLIT3_G24M2O);
 }
-static_assert(cfgeo::kSplitVariants == 11 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
-                  CF_PLAN_SPLIT3_G28M2O == CF_PLAN_SPLIT3_G24M2O + 1 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2O + 1,
+static_assert(cfgeo::kSplitVariants == 13 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
+                  CF_PLAN_SPLIT3_G28M2H == CF_PLAN_SPLIT3_G24M2O + 3 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2H + 1,
               "plan ids of the four-step variants");
 // plans whose passes go through the workspace (one cube per convolution in flight)
 bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
@@ -109,7 +109,7 @@ struct Engine {
   hipModule_t mod = nullptr;
   hipStream_t stream = nullptr;
   hipFunction_t twgen = nullptr, dft16gen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr, filll = nullptr;
-  hipFunction_t batch_run = nullptr, batch_reset = nullptr;
+  hipFunction_t batch_run = nullptr, batch_run_3p = nullptr;
   hipStream_t capture = nullptr;  // private stream for graph capture
   std::map<unsigned, SizeFns> sizes;
   // workspace of the host-buffer (drop-in) calls
@@ -146,6 +146,7 @@ struct Engine {
   // cf_profile_begin/end: events of up to prof_max calls, (CF_MAX_PASSES+1) each
   std::vector<hipEvent_t> prof_ev;
   unsigned prof_max = 0, prof_calls = 0, prof_npass = 0;
+  unsigned prof_every = 1, prof_seen = 0;  // record every prof_every-th call while armed
   std::vector<std::string> prof_names;
   std::vector<double> prof_bytes;
 };
@@ -206,7 +207,7 @@ void release_all() {
   g.stream = nullptr;
   if (g.mod) (void)hipModuleUnload(g.mod);
   g.mod = nullptr;
-  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_reset = nullptr;
+  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_run_3p = nullptr;
   g.scratch = nullptr;  // part of the unloaded module
   g.scratch_bytes = 0;
   if (g.capture) (void)hipStreamDestroy(g.capture);
@@ -306,9 +307,11 @@ int size_res(unsigned n, SizeFns** out) {
       snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
     snprintf(name, sizeof name, "cf_xy1_fwd_%u_g%d%d", n, ga, gb);
     CFCHK(get_fn(&s.xy1_fwd[vi], name));
-    snprintf(name, sizeof name, "cf_xy1_inv_%u_g%d%d", n, ga, gbo);  // reads the middle pass's output layout
+    // the last pass reads the middle pass's output layout: x group gbo, y2 pairs interleaved for "h"
+    const bool ybo2 = cfgeo::split_ybo(vi) == 2;
+    snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : "cf_xy1_inv_%u_g%d%d", n, ga, gbo);
     CFCHK(get_fn(&s.xy1_inv[vi], name));
-    snprintf(name, sizeof name, gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
+    snprintf(name, sizeof name, ybo2 ? "cf_mid_%u_%sh" : gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
     CFCHK(get_fn(&s.mid[vi], name));
     snprintf(name, sizeof name, "cf_midspec_%u_%s", n, sfx);
     CFCHK(get_fn(&s.mspec[vi], name));
@@ -1100,7 +1103,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
   if (get_fn(&g.twgen, "cf_twiddle_gen") != CF_OK || get_fn(&g.dft16gen, "cf_dft16_table_gen") != CF_OK || get_fn(&g.errp, "cf_errnorm_partial") != CF_OK ||
       get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
       get_fn(&g.filll, "cf_fill_loguniform") != CF_OK ||
-      get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_reset, "cf_batch_reset") != CF_OK ||
+      get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_run_3p, "cf_batch_run_3p") != CF_OK ||
       ensure_events() != CF_OK) {
     fprintf(stderr, "convfpga: '%s' is not a convfpga code object (%s)\n", path, g_err.c_str());
     release_all();
@@ -1405,7 +1408,7 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
     }
   } wr{work, s};
   if (!times) {
-    if (g.prof_max > g.prof_calls) {  // armed by cf_profile_begin
+    if (g.prof_max > g.prof_calls && g.prof_seen++ % g.prof_every == 0) {  // armed by cf_profile_begin
       hipEvent_t* ev = &g.prof_ev[(size_t)g.prof_calls * (CF_MAX_PASSES + 1)];
       CFCHK(run_passes(p, n, how_many, s, f->tw, ev));
       if (g.prof_calls == 0) {
@@ -1555,9 +1558,10 @@ extern "C" int cf_conv3d_share_timed(cf_spectrum_t h, const void* d_sig, void* d
   return CF_OK;
 }
 
-extern "C" int cf_profile_begin(unsigned max_calls) {
+extern "C" int cf_profile_begin(unsigned max_calls, unsigned every) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (max_calls == 0 || max_calls > 100000) return fail(CF_ERR_ARG, "max_calls must be in 1..100000");
+  if (every == 0 || every > 1000) return fail(CF_ERR_ARG, "every must be in 1..1000");
   CFCHK(require_ready());
   for (auto e : g.prof_ev) (void)hipEventDestroy(e);
   g.prof_ev.assign((size_t)max_calls * (CF_MAX_PASSES + 1), nullptr);
@@ -1566,6 +1570,8 @@ extern "C" int cf_profile_begin(unsigned max_calls) {
   for (auto& e : g.prof_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   g.prof_max = max_calls;
   g.prof_calls = 0;
+  g.prof_every = every;
+  g.prof_seen = 0;
   return CF_OK;
 }
 
@@ -1692,7 +1698,59 @@ static int batch_capture_end(cf_batch_s* b, int rc) {
   return CF_OK;
 }
 
-extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int scale_mode, int mode, cf_batch_t* out) {
+// Persistent-mode schedule of one instance: its passes (kind, tiles, input,
+// output) on the plan of its spectrum, or CF_ERR_ARG when the persistent kernel
+// does not run that plan.
+struct BatchPass {
+  int kind, variant, ntiles, tile_threads, siblings;
+  const void* in;
+  void* out;
+  const void* H;
+  float scale;
+};
+
+static int batch_schedule(const cf_instance_t& x, unsigned i, void* w1, void* w2, float last_scale,
+                          std::vector<BatchPass>& v) {
+  const unsigned n = x.h->n;
+  const int plan = x.h->plan;
+  const void* sig = x.d_sig;
+  void* out = x.d_out;
+  const void* H = x.h->d;
+  const int rt = cfgeo::row_tiles((int)n), ct = cfgeo::col_tiles((int)n);
+  if (plan == CF_PLAN_AXIS5 && n >= 16 && n <= 256) {
+    v.push_back({CF_BK_ROW_FWD, 0, rt, 256, 1, sig, out, nullptr, 1.0f});
+    v.push_back({CF_BK_COLY_FWD, 0, ct, 256, 1, out, out, nullptr, 1.0f});
+    v.push_back({CF_BK_COLZ_CONV, 0, ct, 256, 1, out, out, H, 1.0f});
+    v.push_back({CF_BK_COLY_INV, 0, ct, 256, 1, out, out, nullptr, 1.0f});
+    v.push_back({CF_BK_ROW_INV, 0, rt, 256, 1, out, out, nullptr, last_scale});
+    return CF_OK;
+  }
+  if (plan == CF_PLAN_PLANE3 && (n == 32 || n == 64)) {
+    const int pt = cfgeo::plane_tiles((int)n), pth = cfgeo::plane_threads((int)n);
+    v.push_back({CF_BK_PLANE_FWD, 0, pt, pth, 1, sig, out, nullptr, 1.0f});
+    v.push_back({CF_BK_COLZ_CONV, 0, ct, 256, 1, out, out, H, 1.0f});
+    v.push_back({CF_BK_PLANE_INV, 0, pt, pth, 1, out, out, nullptr, last_scale});
+    return CF_OK;
+  }
+  if (is_split(plan) && cfgeo::batch_split_supported(split_variant(plan)) && (n == 128 || n == 256)) {
+    const int vi = split_variant(plan), nn = (int)n;
+    const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi), gbo = cfgeo::split_gbo(vi);
+    const int sa = ga * gb >= 16 ? 1 : 16 / (ga * gb), sc = ga * gbo >= 16 ? 1 : 16 / (ga * gbo);
+    v.push_back({CF_BK_XY1_FWD, vi, cfgeo::split_a_tiles(nn, vi), cfgeo::split_a_threads(nn, vi), sa, sig, w1, nullptr,
+                 1.0f});
+    v.push_back({CF_BK_MID, vi, cfgeo::split_b_tiles(nn, vi), cfgeo::split_b_threads(nn, vi), gb / gm, w1, w2, H, 1.0f});
+    v.push_back({CF_BK_XY1_INV, vi, cfgeo::split_a_tiles(nn, vi), cfgeo::split_a_threads(nn, vi), sc, w2, out, nullptr,
+                 last_scale});
+    return CF_OK;
+  }
+  return fail(CF_ERR_ARG,
+              "instance %u: the persistent batch runs the axis plan (N = 16..256), the plane plan (N = 32, 64) and "
+              "the four-step plans split3_g24m2 / split3_g24m2h (N = 128, 256); this spectrum is N = %u, plan %d",
+              i, n, plan);
+}
+
+static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mode, int mode, bool stall_test,
+                        unsigned long long spin_limit, cf_batch_t* out) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!inst || !out || count == 0) return fail(CF_ERR_ARG, "NULL pointer or zero count");
   *out = nullptr;
@@ -1709,9 +1767,8 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
     if (mode == CF_BATCH_PERSISTENT) {
       if (x.h->n < 16 || x.h->n > 256)
         return fail(CF_ERR_SIZE, "instance %u: the persistent batch runs edges 16..256, not %u", i, x.h->n);
-      if (needs_work(x.h->plan))
-        return fail(CF_ERR_ARG, "instance %u: spectrum in the private layout of a four-step or z-blocked plan; "
-                    "create it under CF_PLAN_AXIS5 or CF_PLAN_PLANE3 for a persistent batch", i);
+      std::vector<BatchPass> probe;
+      CFCHK(batch_schedule(x, i, nullptr, nullptr, 1.0f, probe));
     }
   }
   if (mode == CF_BATCH_PERSISTENT) {
@@ -1748,67 +1805,115 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
   b->mode = mode;
   int rc = CF_OK;
   if (mode == CF_BATCH_PERSISTENT) {
-    // segments: pass-major, instance-minor
-    std::vector<CfBatchSeg> segs;
-    int item = 0;
-    // tiles per work item: each item ends with one device-scope release (a
-    // write-back of the XCD's L2), so items are several tiles; measured on
-    // MI355X (64 x 64^3, shared spectrum / the mixed batch): 8 tiles 0.54 /
-    // 0.95 ms, 16 0.48 / 0.92, 32 0.52 / 1.12, 48 0.64 / 1.48
-    int per_item = 16;
-    if (const char* e = getenv("CONVFPGA_BATCH_TILES_PER_ITEM")) per_item = std::max(1, atoi(e));
-    for (int s = 0; s < kBatchKinds; ++s)
-      for (unsigned i = 0; i < count; ++i) {
-        const unsigned n = inst[i].h->n;
-        const bool row = s == 0 || s == 4;
-        CfBatchSeg sg{};
-        sg.in = s == 0 ? inst[i].d_sig : inst[i].d_out;
-        sg.out = inst[i].d_out;
-        sg.H = inst[i].h->d;
-        sg.tw = fns[i]->tw;
-        sg.scale = (s == 4 && scale_mode == CF_SCALE_INV_N3) ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
-        sg.n = (int)n;
-        sg.kind = s;
-        sg.first_item = item;
-        sg.ntiles = row ? cfgeo::row_tiles((int)n) : cfgeo::col_tiles((int)n);
-        sg.per_item = per_item;
-        sg.dep = s == 0 ? -1 : (int)((s - 1) * count + i);
-        sg.dep_need = s == 0 ? 0 : segs[(size_t)(s - 1) * count + i].ntiles;
-        item += (sg.ntiles + per_item - 1) / per_item;
-        segs.push_back(sg);
-      }
-    unsigned long long spin_limit = 1ull << 21;
-    if (const char* t = getenv("CONVFPGA_TEST_BATCH_STALL")) {
-      // test hook: make the last dependency unsatisfiable and the wait short, so
-      // the error path runs to completion without a long stall
-      if (atoi(t) == 1) {
-        segs.back().dep_need += 1;
-        spin_limit = 4096;
+    // workspace: the intermediates of the four-step instances, one after the other
+    std::vector<size_t> woff(count, 0);
+    for (unsigned i = 0; i < count; ++i) {
+      woff[i] = work_need;
+      work_need += work_bytes(inst[i].h->plan, inst[i].h->n, 1, inst[i].d_sig, inst[i].d_out);
+    }
+    if (work_need) {
+      const hipError_t e = hipMalloc(&b->d_work, work_need);
+      if (e != hipSuccess) {
+        batch_free(b);
+        return fail(CF_ERR_NOMEM, "batch workspace: %s", hipGetErrorString(e));
       }
     }
-    const int nsegs = (int)segs.size(), nitems = item, nctr = 2 + nsegs;
-    hipError_t e = hipMalloc(&b->d_segs, sizeof(CfBatchSeg) * segs.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctr, sizeof(unsigned) * nctr);
-    if (e == hipSuccess) e = hipMemcpy(b->d_segs, segs.data(), sizeof(CfBatchSeg) * segs.size(), hipMemcpyHostToDevice);
+    // every instance's passes; segment = one pass of one instance, stage-major
+    std::vector<std::vector<BatchPass>> sched(count);
+    int nstages = 0;
+    for (unsigned i = 0; i < count; ++i) {
+      const unsigned n = inst[i].h->n;
+      void* w1 = (char*)b->d_work + woff[i];
+      void* w2 = w1;  // as conv_passes places the intermediates
+      if (two_intermediates(inst[i].h->plan)) {
+        if (inst[i].d_sig != inst[i].d_out)
+          w1 = inst[i].d_out;
+        else
+          w2 = (char*)w1 + cube_bytes(n);
+      }
+      const float last = scale_mode == CF_SCALE_INV_N3 ? (float)(1.0 / (double)cube_elems(n)) : 1.0f;
+      rc = batch_schedule(inst[i], i, w1, w2, last, sched[i]);
+      if (rc != CF_OK) {
+        batch_free(b);
+        return rc;
+      }
+      nstages = std::max(nstages, (int)sched[i].size());
+    }
+    std::vector<CfBatchSeg> segs;
+    int stage_need[kBatchMaxStages] = {};
+    int item = 0;
+    for (int st = 0; st < nstages; ++st)
+      for (unsigned i = 0; i < count; ++i) {
+        if (st >= (int)sched[i].size()) continue;
+        const BatchPass& q = sched[i][st];
+        CfBatchSeg sg{};
+        sg.in = q.in;
+        sg.out = q.out;
+        sg.H = q.H;
+        sg.tw = fns[i]->tw;
+        sg.scale = q.scale;
+        sg.n = (int)inst[i].h->n;
+        sg.kind = q.kind;
+        sg.variant = q.variant;
+        sg.first_item = item;
+        sg.ntiles = q.ntiles;
+        // a work item: the tiles a work-group runs side by side, at least the
+        // sibling tiles that share lines of an intermediate (one L2)
+        sg.per_item = std::max(kBatchThreads / q.tile_threads, q.siblings);
+        sg.stage = st;
+        if (sg.n >= 32 && sg.ntiles % (kBatchThreads / q.tile_threads) != 0) {  // tiles run in whole groups
+          batch_free(b);
+          return fail(CF_ERR_RUNTIME, "pass kind %d of N = %d: %d tiles do not fill work-groups", q.kind, sg.n,
+                      sg.ntiles);
+        }
+        item += (sg.ntiles + sg.per_item - 1) / sg.per_item;
+        stage_need[st] += sg.ntiles;
+        segs.push_back(sg);
+      }
+    if (stall_test) {
+      // test support: the last stage waits for one tile more than the one
+      // before it has, so its wait runs into the (short) spin limit
+      if (nstages < 2) {
+        batch_free(b);
+        return fail(CF_ERR_ARG, "a stall test needs a schedule of at least two passes");
+      }
+      stage_need[nstages - 2] += 1;
+    }
+    const int nsegs = (int)segs.size(), nitems = item;
+    const size_t seg_bytes = sizeof(CfBatchSeg) * segs.size();
+    hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctr, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);
+    if (e == hipSuccess) e = hipMemcpy(b->d_segs, segs.data(), seg_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy((char*)b->d_segs + seg_bytes, stage_need, sizeof stage_need, hipMemcpyHostToDevice);
+    // the counters start at zero once; the kernel leaves them at zero for the next launch
+    if (e == hipSuccess) e = hipMemset(b->d_ctr, 0, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);
     if (e != hipSuccess) {
       batch_free(b);
       return fail(CF_ERR_NOMEM, "batch descriptor allocation: %s", hipGetErrorString(e));
     }
+    // the kernel specialised to the three-pass set when every instance is in it
+    bool three_pass = true;
+    for (unsigned i = 0; i < count; ++i) {
+      const unsigned n = inst[i].h->n;
+      const int p = inst[i].h->plan;
+      three_pass = three_pass && ((p == CF_PLAN_PLANE3 && (n == 32 || n == 64)) ||
+                                  (is_split(p) && split_variant(p) == cfgeo::kBatchSplit[1] && (n == 128 || n == 256)));
+    }
+    hipFunction_t kfn = three_pass ? g.batch_run_3p : g.batch_run;
     int per_cu = 1, cus = 1;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.batch_run, kBatchThreads, 0) != hipSuccess ||
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBatchThreads, 0) != hipSuccess ||
         per_cu < 1)
       per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device) != hipSuccess || cus < 1) cus = 1;
     const unsigned grid = (unsigned)std::min(nitems, per_cu * cus);
     HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
     unsigned* ctr = b->d_ctr;
-    int nc = nctr;
-    void* ra[] = {&ctr, &nc};
     const void* sp = b->d_segs;
+    const void* need = (const char*)b->d_segs + seg_bytes;
     int ns = nsegs, ni = nitems;
-    void* ba[] = {(void*)&sp, &ns, &ni, &ctr, &spin_limit};
-    if (hipModuleLaunchKernel(g.batch_reset, 1, 1, 1, 256, 1, 1, 0, g.capture, ra, nullptr) != hipSuccess ||
-        hipModuleLaunchKernel(g.batch_run, grid, 1, 1, kBatchThreads, 1, 1, 0, g.capture, ba, nullptr) != hipSuccess)
+    unsigned long long sl = spin_limit;
+    void* ba[] = {(void*)&sp, (void*)&need, &ns, &ni, &ctr, &sl};
+    if (hipModuleLaunchKernel(kfn, grid, 1, 1, kBatchThreads, 1, 1, 0, g.capture, ba, nullptr) != hipSuccess)
       rc = fail(CF_ERR_RUNTIME, "batch kernel capture failed");
     rc = batch_capture_end(b, rc);
   } else {
@@ -1866,6 +1971,16 @@ extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int sc
   return CF_OK;
 }
 
+extern "C" int cf_batch_create(const cf_instance_t* inst, unsigned count, int scale_mode, int mode, cf_batch_t* out) {
+  return batch_create(inst, count, scale_mode, mode, false, 1ull << 21, out);
+}
+
+extern "C" int cf_batch_create_stall_test(const cf_instance_t* inst, unsigned count, int scale_mode,
+                                          unsigned long long spin_limit, cf_batch_t* out) {
+  if (spin_limit == 0) return fail(CF_ERR_ARG, "spin_limit must be at least 1");
+  return batch_create(inst, count, scale_mode, CF_BATCH_PERSISTENT, true, spin_limit, out);
+}
+
 extern "C" int cf_batch_launch(cf_batch_t b, void* stream) {
   std::lock_guard<std::recursive_mutex> lk(g.mu);
   if (!b || !b->exec) return fail(CF_ERR_ARG, "NULL batch");
@@ -1884,8 +1999,12 @@ extern "C" int cf_batch_check(cf_batch_t b) {
   HIPCHK(hipStreamSynchronize(b->last));
   if (b->mode == CF_BATCH_PERSISTENT) {
     unsigned flag = 0;
-    HIPCHK(hipMemcpy(&flag, b->d_ctr + 1, sizeof flag, hipMemcpyDeviceToHost));
-    if (flag) return fail(CF_ERR_RUNTIME, "persistent batch: a pass dependency was not met in time");
+    HIPCHK(hipMemcpy(&flag, b->d_ctr + kBatchCtrStride, sizeof flag, hipMemcpyDeviceToHost));  // counter 1
+    if (flag) {
+      const unsigned zero = 0;  // the next launch starts clean
+      HIPCHK(hipMemcpy(b->d_ctr + kBatchCtrStride, &zero, sizeof zero, hipMemcpyHostToDevice));
+      return fail(CF_ERR_RUNTIME, "persistent batch: a stage's dependency was not met in time");
+    }
   }
   return CF_OK;
 }

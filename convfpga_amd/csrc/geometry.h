@@ -39,23 +39,34 @@ constexpr int plane_tiles(int n) { return n; }
 // divides GB), GBO = x group of the intermediate the middle pass writes and
 // the last pass reads: GB (in place) or GM (out of place, every tile's output
 // one contiguous block).
-constexpr int kSplitVariants = 11;
-//                                      g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o
-constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2};
-constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8};
-constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2};
-constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2};
+constexpr int kSplitVariants = 13;
+//                                       g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o g24m2h g28m2h
+constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8, 4, 8};
+constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2, 2, 2};
+constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2, 4, 8};
+// y2 values interleaved into each run of the second intermediate (YBO = 2:
+// middle tiles of 2 x write runs of 4 points, last-pass tiles of 2 y2 read
+// runs of 2*GBO points)
+constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2};
 constexpr int split_ga(int v) { return kSplitGA[v]; }
 constexpr int split_gb(int v) { return kSplitGB[v]; }
 constexpr int split_gm(int v) { return kSplitGM[v]; }
 constexpr int split_gbo(int v) { return kSplitGBO[v]; }
+constexpr int split_ybo(int v) { return kSplitYBO[v]; }
 // the middle pass writes a second intermediate (the last pass reads that one)
-constexpr bool split_out_of_place(int v) { return kSplitGBO[v] != kSplitGB[v]; }
+constexpr bool split_out_of_place(int v) { return kSplitGBO[v] != kSplitGB[v] || kSplitYBO[v] != 1; }
 constexpr bool split_supported(int n, int v) { return (n == 128 || n == 256) && v >= 0 && v < kSplitVariants; }
 constexpr int split_a_threads(int n, int v) { return n * split_ga(v); }
 constexpr int split_a_tiles(int n, int v) { return n * ((n / 16) / split_ga(v)); }
 constexpr int split_b_threads(int n, int v) { return (n / 16) * split_gm(v) * (n / 16); }
 constexpr int split_b_tiles(int n, int v) { return 16 * (n / split_gm(v)); }
+
+// four-step variants the persistent batch kernel runs (outer tiles of 2 y2
+// values with one row per thread, middle tiles of 2 x): g24m2 and g24m2h
+constexpr int kBatchSplitCount = 2;
+constexpr int kBatchSplit[kBatchSplitCount] = {4, 11};
+constexpr bool batch_split_supported(int v) { return v == kBatchSplit[0] || v == kBatchSplit[1]; }
 
 // device scratch of the allocation-free entry point (cf_scratch in the code
 // object): one 256^3 cube, the four-step intermediate of one convolution

@@ -29,6 +29,7 @@ struct ColCfg {
 template <int N>
 struct RowCfg {
   static constexpr int P = Shape<N>::P, T = N / P;
+  static_assert(64 % T == 0, "the threads of a row are lanes of one wave");
   static constexpr int L0 = 256 / T;
   static constexpr int L = L0 > N * N ? N * N : L0;  // rows per tile
   static constexpr int THREADS = L * T;
@@ -62,7 +63,7 @@ struct RowPass {
     for (int m = 0; m < P; ++m) r.v[m] = src[T * m];
   }
   __device__ __forceinline__ void process(int tile, Regs& r) const {
-    fft_line<N, INV>(r.v, t, RowLds{lds + l * K::PITCH}, tw_s);
+    fft_line<N, INV, true>(r.v, t, RowLds{lds + l * K::PITCH}, tw_s);  // a row: lanes of one wave
     float2* dst = out + base(tile);
 #pragma unroll
     for (int m = 0; m < P; ++m) dst[T * m] = SCALE ? cscale(r.v[m], scale) : r.v[m];

@@ -1,168 +1,398 @@
 // batch_kernel.h -- one persistent launch for a list of convolutions of mixed
-// sizes, each with its own (or a shared) filter spectrum.
+// sizes, each with its own (or a shared) filter spectrum, on each instance's
+// own schedule: the plane plan (N = 32, 64), the four-step plans (N = 128,
+// 256) or the five-pass axis plan (16..256).
 //
 // The reference overlaps the phases of consecutive volumes by software
 // pipelining pairs of instances on its FPGA queues
 // (host/src/convfpga_svm_batch.c:241-662).  Here every pass tile of every
-// instance is one work item of a single queue:
+// instance is work of a single queue, ordered by stage (the pass's index in
+// its instance's schedule): all tiles of stage 0 of every instance, then all
+// tiles of stage 1, ...  A work-group takes the next work item (a few
+// consecutive tiles of one pass) with an atomic counter and runs it with the
+// ordinary pass code.
 //
-//   items are ordered pass-major: all tiles of pass 0 of every instance, then
-//   all tiles of pass 1, ... (5 passes: x, y, z*H*z, y^-1, x^-1)
-//   a work-group takes the next item with an atomic counter, waits until the
-//   previous pass of the same instance has completed all its tiles, runs the
-//   tile with the ordinary pass code, then publishes its completion.
+// Ordering between the stages, one cross-XCD release per work-group per stage:
+// when a work-group takes its first item of a later stage, its waves wait for
+// their own stores, then one thread releases (device scope: the XCD's L2 is
+// written back once for everything the work-group stored in that stage) and
+// adds its count of completed tiles to the stage's counter; it then waits
+// until the previous stage is complete and acquires (device scope) before any
+// wave loads.  No deadlock: a work-group waiting at a stage boundary has
+// published its own tiles, and every tile of the previous stage was taken
+// before any tile of the later one, by work-groups that are running and never
+// wait inside a stage.  The wait is bounded: after `spin_limit` polls it raises
+// the error flag and goes on, so the grid always drains (the host reports it).
 //
-// No deadlock is possible: an item only ever waits for items that come
-// earlier in the queue, and those were taken by work-groups that are already
-// running and never wait for later items.  The wait is still bounded: after
-// `spin_limit` polls (the host passes 2^21, a few seconds) it raises the error flag
-// and proceeds, so the grid always drains (the host reports the flag).
+// A work-group has 512 threads.  A pass whose tile needs T < 512 threads runs
+// 512/T tiles side by side, each group of T threads on its own slice of LDS;
+// they execute the same code, barriers included, so the host makes every
+// work item a multiple of 512/T tiles (edges of 16, whose passes have no
+// barrier, may leave a group idle).
 //
-// Completion is published with a device-scope release (every thread's tile
-// stores are fenced first) and observed with a device-scope acquire, which on
-// gfx950 writes back / invalidates the per-XCD L2 as needed for another XCD to
-// see the data.
+// The counters reset themselves: the last work-group to finish zeroes the
+// queue head and the stage counters for the next launch (the error flag is
+// cleared by the host when it reads it), so a captured graph needs no memset.
 #pragma once
 #include "../batch_desc.h"
+#include "../geometry.h"
 #include "axis_passes.h"
+#include "three_pass.h"
 
 namespace cfk {
 
-enum BatchPassKind : int { BK_ROW_FWD = 0, BK_COLY_FWD = 1, BK_COLZ_CONV = 2, BK_COLY_INV = 3, BK_ROW_INV = 4 };
+// four-step variants the persistent kernel runs (outer tiles of 2 y2 values,
+// one row per thread; middle tiles of 2 x), as tile shapes
+template <int V>
+struct BatchSplit {
+  static constexpr int GA = cfgeo::split_ga(V), GB = cfgeo::split_gb(V), GM = cfgeo::split_gm(V);
+  static constexpr int GBO = cfgeo::split_gbo(V), YBO = cfgeo::split_ybo(V);
+  static_assert(GA == 2 && GM == 2, "persistent four-step variants: outer tiles of 2 y2, middle tiles of 2 x");
+};
 
-// ctr[0]: queue head (work items), ctr[1]: error flag, ctr[2 + s]: completed tiles of segment s
-// (kept out of line: inlining every size into one switch makes the register
-// allocator take far more VGPRs than any single size needs)
+// LDS (float2) of one work item: every tile run side by side
+template <int THREADS, int LDS_TILE>
+struct BatchLds {
+  static constexpr int value = (kBatchThreads / THREADS) * LDS_TILE;
+};
 template <int N>
-__device__ __noinline__ void batch_tile(const float2* in, float2* out, const float2* H, float scale, int kind, int tile,
-                                        float2* lds, const float2* tw_s) {
+constexpr int batch_axis_lds() {
+  return BatchLds<256, (RowCfg<N>::LDS > ColCfg<N>::LDS ? RowCfg<N>::LDS : ColCfg<N>::LDS)>::value;
+}
+template <int N>
+constexpr int batch_plane_lds() {
+  return BatchLds<PlaneCfg<N>::THREADS, PlaneCfg<N>::LDS>::value;
+}
+template <int N, int V>
+constexpr int batch_split_lds() {
+  using B = BatchSplit<V>;
+  using A = SplitGeo<N, 2, B::GB>;
+  using M = MidGeoN<N, B::GB, 2>;
+  constexpr int a = BatchLds<A::A_THREADS, A::A_LDS>::value, m = BatchLds<M::THREADS, M::LDS>::value;
+  return a > m ? a : m;
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kBatchLds =
+    cmax(cmax(cmax(batch_axis_lds<256>(), batch_axis_lds<128>()), cmax(batch_plane_lds<32>(), batch_plane_lds<64>())),
+         cmax(cmax(batch_split_lds<256, cfgeo::kBatchSplit[0]>(), batch_split_lds<256, cfgeo::kBatchSplit[1]>()),
+              cmax(batch_split_lds<128, cfgeo::kBatchSplit[0]>(), batch_split_lds<128, cfgeo::kBatchSplit[1]>())));
+static_assert(cfgeo::kBatchSplitCount == 2, "the LDS bound above covers two four-step variants");
+
+struct BatchTile {
+  const float2* in;
+  float2* out;
+  const float2* H;
+  float scale;
+  int tile;  // tile index within the instance's cube
+  int tid;   // thread within the tile's group
+  float2* lds;  // the group's LDS slice
+  const float2* tw_s;
+};
+
+// The tile bodies.  The general kernel calls them out of line, one function
+// per kind and size: inlining every shape into one switch makes the register
+// allocator spill (more VGPRs than any single pass needs); the calls cost a
+// save and restore of the callee's registers per tile.  The kernel of the
+// three-pass plans inlines the few shapes it runs.
+template <int N>
+__device__ __forceinline__ void batch_axis_tile_body(int kind, const BatchTile& t) {
   if constexpr (N >= 16) {
-    if (kind == BK_ROW_FWD || kind == BK_ROW_INV) {
+    if (kind == CF_BK_ROW_FWD || kind == CF_BK_ROW_INV) {
       using K = RowCfg<N>;
-      static_assert(K::THREADS == 256, "batch kernel runs 256 threads");
-      const int t = threadIdx.x % K::T, l = threadIdx.x / K::T;
-      if (kind == BK_ROW_FWD) {
-        const RowPass<N, false, false> k{in, out, tw_s, lds, scale, t, l};
+      const int tt = t.tid % K::T, l = t.tid / K::T;
+      if (kind == CF_BK_ROW_FWD) {
+        const RowPass<N, false, false> k{t.in, t.out, t.tw_s, t.lds, t.scale, tt, l};
         typename RowPass<N, false, false>::Regs r;
-        k.load(tile, r);
-        k.process(tile, r);
+        k.load(t.tile, r);
+        k.process(t.tile, r);
       } else {
-        const RowPass<N, true, true> k{in, out, tw_s, lds, scale, t, l};
+        const RowPass<N, true, true> k{t.in, t.out, t.tw_s, t.lds, t.scale, tt, l};
         typename RowPass<N, true, true>::Regs r;
-        k.load(tile, r);
-        k.process(tile, r);
+        k.load(t.tile, r);
+        k.process(t.tile, r);
       }
     } else {
       using K = ColCfg<N>;
-      static_assert(K::THREADS == 256, "batch kernel runs 256 threads");
-      const int c = threadIdx.x % K::C, t = (threadIdx.x / K::C) % K::T, g = threadIdx.x / (K::C * K::T);
-      if (kind == BK_COLY_FWD) {
-        const ColPass<N, 1, 0> k{in, out, H, tw_s, lds, 0, c, t, g};
+      const int c = t.tid % K::C, tt = (t.tid / K::C) % K::T, g = t.tid / (K::C * K::T);
+      if (kind == CF_BK_COLY_FWD) {
+        const ColPass<N, 1, 0> k{t.in, t.out, t.H, t.tw_s, t.lds, 0, c, tt, g};
         typename ColPass<N, 1, 0>::Regs r;
-        k.load(tile, r);
-        k.process(tile, r);
-      } else if (kind == BK_COLZ_CONV) {
-        const ColPass<N, 2, 2> k{in, out, H, tw_s, lds, 0, c, t, g};
+        k.load(t.tile, r);
+        k.process(t.tile, r);
+      } else if (kind == CF_BK_COLZ_CONV) {
+        const ColPass<N, 2, 2> k{t.in, t.out, t.H, t.tw_s, t.lds, 0, c, tt, g};
         typename ColPass<N, 2, 2>::Regs r;
-        k.load(tile, r);
-        k.process(tile, r);
+        k.load(t.tile, r);
+        k.process(t.tile, r);
       } else {
-        const ColPass<N, 1, 1> k{in, out, H, tw_s, lds, 0, c, t, g};
+        const ColPass<N, 1, 1> k{t.in, t.out, t.H, t.tw_s, t.lds, 0, c, tt, g};
         typename ColPass<N, 1, 1>::Regs r;
-        k.load(tile, r);
-        k.process(tile, r);
+        k.load(t.tile, r);
+        k.process(t.tile, r);
       }
     }
   }
 }
 
-constexpr int kBatchLds = RowCfg<256>::LDS > ColCfg<256>::LDS ? RowCfg<256>::LDS : ColCfg<256>::LDS;
-__device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs, int nsegs, int nitems,
-                                             unsigned* ctr, unsigned long long spin_limit) {
+template <int N, bool INV>
+__device__ __forceinline__ void batch_plane_tile_body(const BatchTile& t) {
+  using K = PlaneCfg<N>;
+  const PlanePass<N, INV> k{t.in, t.out, t.tw_s, t.lds, t.scale, t.tid % K::T, t.tid / K::T, t.tid % N, t.tid / N};
+  typename PlanePass<N, INV>::Regs r;
+  k.load(t.tile, r);
+  k.process(t.tile, r);
+}
+
+template <int N, int V, bool INV>
+__device__ __forceinline__ void batch_xy1_tile_body(const BatchTile& t) {
+  using B = BatchSplit<V>;
+  constexpr int GB = INV ? B::GBO : B::GB, YB = INV ? B::YBO : 1;
+  using K = SplitGeo<N, 2, GB, YB>;
+  using P = Xy1Pass<N, 2, GB, INV, YB>;
+  const int x = t.tid, row = x / K::T;
+  const int cxl = YB == 1 ? x % GB : (x / YB) % GB;
+  const int cy2l = YB == 1 ? (x / GB) % 2 : x % YB;
+  const int cxh = x / (GB * 2);
+  const P k{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, false};
+  typename P::Regs r;
+  k.load(t.tile, r);
+  k.process(t.tile, r);
+}
+
+template <int N, int V>
+__device__ __forceinline__ void batch_mid_tile_body(const BatchTile& t) {
+  using B = BatchSplit<V>;
+  using M = MidGeoN<N, B::GB, 2>;
+  using P = MidPass<N, 2, B::GB, 2, false, B::GBO, B::YBO>;
+  const P k{t.in, t.out, t.H, t.tw_s, t.lds, 0, t.tid % M::LINES, t.tid / M::LINES, t.tid, false};
+  typename P::Regs r;
+  k.load(t.tile, r);
+  k.process(t.tile, r);
+}
+
+template <int N>
+__device__ __noinline__ void batch_axis_tile(int kind, BatchTile t) {
+  batch_axis_tile_body<N>(kind, t);
+}
+template <int N, bool INV>
+__device__ __noinline__ void batch_plane_tile(BatchTile t) {
+  batch_plane_tile_body<N, INV>(t);
+}
+template <int N, int V, bool INV>
+__device__ __noinline__ void batch_xy1_tile(BatchTile t) {
+  batch_xy1_tile_body<N, V, INV>(t);
+}
+template <int N, int V>
+__device__ __noinline__ void batch_mid_tile(BatchTile t) {
+  batch_mid_tile_body<N, V>(t);
+}
+
+template <int N, int V, bool CALLS>
+__device__ __forceinline__ void batch_split_tile(int kind, const BatchTile& t) {
+  if (kind == CF_BK_XY1_FWD) {
+    if constexpr (CALLS) batch_xy1_tile<N, V, false>(t); else batch_xy1_tile_body<N, V, false>(t);
+  } else if (kind == CF_BK_MID) {
+    if constexpr (CALLS) batch_mid_tile<N, V>(t); else batch_mid_tile_body<N, V>(t);
+  } else {
+    if constexpr (CALLS) batch_xy1_tile<N, V, true>(t); else batch_xy1_tile_body<N, V, true>(t);
+  }
+}
+
+// threads of one tile of (kind, n): 512 / that = tiles run side by side
+__device__ __forceinline__ int batch_tile_threads(int kind, int n) {
+  switch (kind) {
+    case CF_BK_PLANE_FWD:
+    case CF_BK_PLANE_INV: return cfgeo::plane_threads(n);
+    case CF_BK_XY1_FWD:
+    case CF_BK_XY1_INV: return n * 2;
+    case CF_BK_MID: return (n / 16) * 2 * (n / 16);
+    default: return 256;
+  }
+}
+// LDS (float2) of one tile of (kind, n, variant)
+template <int N>
+__device__ __forceinline__ int batch_tile_lds(int kind) {
+  switch (kind) {
+    case CF_BK_PLANE_FWD:
+    case CF_BK_PLANE_INV:
+      if constexpr (N == 32 || N == 64) return PlaneCfg<N>::LDS;
+      return 0;
+    case CF_BK_XY1_FWD:
+    case CF_BK_XY1_INV:
+    case CF_BK_MID:
+      if constexpr (N == 128 || N == 256) {
+        // every supported variant has middle tiles of 2 x and outer tiles of 2 y2: the
+        // outer passes' LDS does not depend on the x group, the middle pass's on N only
+        return kind == CF_BK_MID ? MidGeoN<N, 2, 2>::LDS : SplitGeo<N, 2, 2>::A_LDS;
+      }
+      return 0;
+    default:
+      return RowCfg<N>::LDS > ColCfg<N>::LDS ? RowCfg<N>::LDS : ColCfg<N>::LDS;
+  }
+}
+
+// GENERAL: every kind and size the host schedules, tiles called out of line.
+// Otherwise the three-pass set, inlined: the plane plan at 32 and 64 (plane
+// passes and the z pass) and the four-step plan kBatchSplit[1] at 128 and 256.
+template <int N, bool GENERAL>
+__device__ __forceinline__ void batch_dispatch_n(int kind, int variant, const BatchTile& t) {
+  if (kind <= CF_BK_ROW_INV) {
+    if constexpr (GENERAL) {
+      batch_axis_tile<N>(kind, t);
+    } else if constexpr (N == 32 || N == 64) {
+      batch_axis_tile_body<N>(CF_BK_COLZ_CONV, t);  // the only axis kind of the plane plan
+    }
+  } else if (kind == CF_BK_PLANE_FWD || kind == CF_BK_PLANE_INV) {
+    if constexpr (N == 32 || N == 64) {
+      if (kind == CF_BK_PLANE_FWD) {
+        if constexpr (GENERAL) batch_plane_tile<N, false>(t); else batch_plane_tile_body<N, false>(t);
+      } else {
+        if constexpr (GENERAL) batch_plane_tile<N, true>(t); else batch_plane_tile_body<N, true>(t);
+      }
+    }
+  } else {
+    if constexpr (N == 128 || N == 256) {
+      if constexpr (GENERAL) {
+        if (variant == cfgeo::kBatchSplit[0])
+          batch_split_tile<N, cfgeo::kBatchSplit[0], true>(kind, t);
+        else
+          batch_split_tile<N, cfgeo::kBatchSplit[1], true>(kind, t);
+      } else {
+        batch_split_tile<N, cfgeo::kBatchSplit[1], false>(kind, t);
+      }
+    }
+  }
+}
+
+template <int N, bool GENERAL>
+__device__ __forceinline__ void batch_item(const CfBatchSeg& sg, int t0, int t1, float2* lds, const float2* tw_s) {
+  const int tt = batch_tile_threads(sg.kind, N);
+  const int tid = (int)threadIdx.x;
+  // a tile's group of threads is whole waves (tt is a multiple of 64): the
+  // group, and with it the tile, is uniform in a wave -- scalar registers,
+  // buffer descriptors without waterfall loops
+  const int sub = kBatchThreads / tt, g = __builtin_amdgcn_readfirstlane(tid / tt);
+  BatchTile t{(const float2*)sg.in, (float2*)sg.out, (const float2*)sg.H, sg.scale, 0, tid % tt,
+              lds + g * batch_tile_lds<N>(sg.kind), tw_s};
+  for (int tile = t0; tile < t1; tile += sub) {
+    // the tiles before may still use the LDS: the pass code ends with exchanges
+    // that only order a wave's own accesses (row FFTs), so a barrier separates
+    // consecutive tiles run by the same threads
+    if (tile > t0) __syncthreads();
+    t.tile = __builtin_amdgcn_readfirstlane(tile + g);
+    // groups past the end are idle only for edges of 16, whose passes have no barrier
+    if (t.tile < t1) batch_dispatch_n<N, GENERAL>(sg.kind, sg.variant, t);
+  }
+}
+
+template <bool GENERAL>
+__device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs, const int* __restrict__ stage_need,
+                                             int nsegs, int nitems, unsigned* ctr, unsigned long long spin_limit) {
   __shared__ float2 lds[kBatchLds];
   __shared__ float2 tw_s[256];
-  __shared__ int s_item, s_seg;
-  // Control flow note: thread 0 works alone in exactly one place, at the top of
-  // the body (publish the previous tile, take the next item, wait for its
-  // dependency); everything else runs on all threads with uniform branches, and
-  // the loop's back-edge follows a barrier.  A second single-thread region at
-  // the bottom of the body lets the compiler peel an exec-masked inner loop for
-  // the other lanes, in which they pass the top barrier again without thread 0
-  // and spin on a stale item.
-  int prev = -1;  // segment of the item just finished (uniform)
-  int prev_tiles = 0;
+  __shared__ int s_item, s_seg, s_stage;
+  unsigned* const head = ctr;                                // queue head
+  unsigned* const error = ctr + 1 * kBatchCtrStride;         // error flag
+  unsigned* const finished = ctr + 2 * kBatchCtrStride;      // work-groups finished
+  auto stage_ctr = [&](int s) { return ctr + (3 + s) * kBatchCtrStride; };  // tiles of stage s completed
+  // Control flow note: thread 0 works alone only between two barriers (taking
+  // the item; publishing and waiting at a stage change) and after the loop;
+  // everything else runs on all threads with uniform branches.
+  int tw_n = 0;             // edge whose twiddles are in tw_s
+  int cur_stage = -1;       // stage of the tiles run since the last publication
+  unsigned done_tiles = 0;  // how many (uniform)
+  // thread 0 fetches each next item while the current one runs (the atomic's
+  // latency hidden behind the tiles)
+  int next = threadIdx.x == 0 ? (int)atomicAdd(head, 1u) : 0;
   for (;;) {
+    // (no barrier at the end of an item: this one, after thread 0's region,
+    // also frees the LDS and the twiddles of the previous item)
     if (threadIdx.x == 0) {
-      if (prev >= 0)
-        __hip_atomic_fetch_add(&ctr[2 + prev], (unsigned)prev_tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      const int item = (int)atomicAdd(&ctr[0], 1u);
-      int seg = -1;
-      if (item < nitems) {  // segments are sorted by first_item: binary search
-        int lo = 0, hi = nsegs - 1;
+      const int item = next;
+      int seg = -1, stage = kBatchMaxStages;
+      if (item < nitems) {
+        next = (int)atomicAdd(head, 1u);
+        int lo = 0, hi = nsegs - 1;  // segments are sorted by first_item: binary search
         while (lo < hi) {
           const int mid = (lo + hi + 1) / 2;
           if (segs[mid].first_item <= item) lo = mid; else hi = mid - 1;
         }
         seg = lo;
-        const int dep = segs[seg].dep;
-        if (dep >= 0) {
-          const unsigned need = (unsigned)segs[seg].dep_need;
-          unsigned long long spins = 0;
-          // relaxed polls (no cache invalidation per poll); the acquire fence
-          // after the barrier below orders the tile's loads after the wait
-          while (__hip_atomic_load(&ctr[2 + dep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-            __builtin_amdgcn_s_sleep(8);
-            if (++spins > spin_limit) {
-              atomicOr(&ctr[1], 1u);
-              break;
-            }
-          }
-        }
-        // one acquire for the work-group: it invalidates the caches this CU
-        // reads through (waited for, so it is complete before any wave loads);
-        // the other waves follow it through the barrier below
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __builtin_amdgcn_s_waitcnt(0);
+        stage = segs[seg].stage;
       }
       s_item = item;
       s_seg = seg;
+      s_stage = stage;
     }
     __syncthreads();
     // every lane reads the same values; readfirstlane lets the compiler see it
     const int item = __builtin_amdgcn_readfirstlane(s_item);
     const int seg = __builtin_amdgcn_readfirstlane(s_seg);
+    const int stage = __builtin_amdgcn_readfirstlane(s_stage);
+    if (stage != cur_stage) {
+      // leaving cur_stage: every wave's stores have reached its L2, then one
+      // device-scope release and one count for the whole work-group
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (cur_stage >= 0 && done_tiles > 0)
+          __hip_atomic_fetch_add(stage_ctr(cur_stage), done_tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (item < nitems && stage > 0) {
+          // the previous stage of every instance complete (which implies the earlier ones)
+          const unsigned need = (unsigned)stage_need[stage - 1];
+          unsigned long long spins = 0;
+          // relaxed polls (no cache invalidation per poll), a few hundred
+          // nanoseconds apart; the acquire below orders the tiles' loads after the wait
+          while (__hip_atomic_load(stage_ctr(stage - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            __builtin_amdgcn_s_sleep(16);
+            if (++spins > spin_limit) {
+              atomicOr(error, 1u);
+              break;
+            }
+          }
+          // one acquire for the work-group: it invalidates the caches this CU
+          // reads through; the other waves follow through the barrier below
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          __builtin_amdgcn_s_waitcnt(0);
+        }
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      cur_stage = stage;
+      done_tiles = 0;
+    }
     if (item >= nitems) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // ordered after thread 0's acquire
     const CfBatchSeg sg = segs[seg];
-    for (int i = threadIdx.x; i < sg.n; i += 256) tw_s[i] = ((const float2*)sg.tw)[i];
-    __syncthreads();
-    // a work item is a run of up to per_item consecutive tiles of the segment:
-    // one release per item instead of per tile (each device-scope release
-    // writes back the XCD's L2)
+    if (sg.n != tw_n) {  // the twiddles of the item's edge (uniform); kept while the edge stays
+      for (int i = threadIdx.x; i < sg.n && i < 256; i += kBatchThreads) tw_s[i] = ((const float2*)sg.tw)[i];
+      __syncthreads();
+      tw_n = sg.n;
+    }
     const int t0 = (item - sg.first_item) * sg.per_item;
     const int t1 = min(t0 + sg.per_item, sg.ntiles);
-    const float2* in = (const float2*)sg.in;
-    float2* out = (float2*)sg.out;
-    const float2* H = (const float2*)sg.H;
-    for (int tile = t0; tile < t1; ++tile) {
-      switch (sg.n) {
-        case 16: batch_tile<16>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-        case 32: batch_tile<32>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-        case 64: batch_tile<64>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-        case 128: batch_tile<128>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-        case 256: batch_tile<256>(in, out, H, sg.scale, sg.kind, tile, lds, tw_s); break;
-        default: break;
-      }
+    switch (sg.n) {
+      case 16:
+        if constexpr (GENERAL) batch_item<16, GENERAL>(sg, t0, t1, lds, tw_s);
+        break;
+      case 32: batch_item<32, GENERAL>(sg, t0, t1, lds, tw_s); break;
+      case 64: batch_item<64, GENERAL>(sg, t0, t1, lds, tw_s); break;
+      case 128: batch_item<128, GENERAL>(sg, t0, t1, lds, tw_s); break;
+      case 256: batch_item<256, GENERAL>(sg, t0, t1, lds, tw_s); break;
+      default: break;
     }
-    // every wave waits until its stores have reached L2 (s_waitcnt 0: all
-    // counters), then thread 0's device-scope release at the top of the next
-    // turn writes the XCD's L2 back once for the whole item instead of once
-    // per wave
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();  // LDS free again
-    prev = seg;
-    prev_tiles = t1 - t0;
+    done_tiles += (unsigned)(t1 - t0);
+  }
+  // the last work-group out resets the queue and the stage counters for the
+  // next launch (every other work-group has stopped using them; the release
+  // orders this work-group's own queue atomics before its count)
+  if (threadIdx.x == 0) {
+    const unsigned out = __hip_atomic_fetch_add(finished, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (out == gridDim.x - 1) {
+      __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int s = 0; s < kBatchMaxStages; ++s) __hip_atomic_store(stage_ctr(s), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(finished, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 

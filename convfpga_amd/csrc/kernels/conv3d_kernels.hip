@@ -116,6 +116,15 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, GM>::THREADS)) void cf_mid_##N##_##SUFFIX(CF_ARGS) { \
     cfk::mid_kernel<N, GA, GB, GM, (GM <= 2), GBO>(in, out, H, tw, hstride, ntotal);                            \
   }
+// middle pass that writes the input's layout with y2 pairs interleaved, and
+// the inverse outer pass that reads it
+#define CF_SPLIT_H(N, GB, SUFFIX)                                                                               \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, GB, 2>::THREADS)) void cf_mid_##N##_##SUFFIX(CF_ARGS) { \
+    cfk::mid_kernel<N, 2, GB, 2, true, GB, 2>(in, out, H, tw, hstride, ntotal);                                \
+  }                                                                                                             \
+  extern "C" __global__ __launch_bounds__(N * 2) void cf_xy1_inv_##N##_g2##GB##h(CF_ARGS) {                     \
+    cfk::xy1_kernel<N, 2, GB, true, 2>(in, out, tw, scale, ntotal);                                            \
+  }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
 // (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
@@ -135,6 +144,8 @@ CF_SPLIT_B(256, 2, 8, 2, g28m2)
 CF_SPLIT_B(256, 2, 8, 1, g28m1)
 CF_SPLIT_BO(256, 2, 4, 2, 2, g24m2o)
 CF_SPLIT_BO(256, 2, 8, 2, 2, g28m2o)
+CF_SPLIT_H(256, 4, g24m2h)
+CF_SPLIT_H(256, 8, g28m2h)
 CF_SPLIT_AC(128, 2, 2)
 CF_SPLIT_AC(128, 2, 4)
 CF_SPLIT_AC(128, 4, 2)
@@ -151,6 +162,8 @@ CF_SPLIT_B(128, 2, 8, 2, g28m2)
 CF_SPLIT_B(128, 2, 8, 1, g28m1)
 CF_SPLIT_BO(128, 2, 4, 2, 2, g24m2o)
 CF_SPLIT_BO(128, 2, 8, 2, 2, g28m2o)
+CF_SPLIT_H(128, 4, g24m2h)
+CF_SPLIT_H(128, 8, g28m2h)
 template <int N, int V>
 struct SplitGeoCheck {
   using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;
@@ -192,15 +205,18 @@ extern "C" __global__ void cf_dft16_table_gen(void* table) {
 }
 
 // ---- persistent batch of mixed sizes (batch_kernel.h) -----------------------------
-extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(
-    const CfBatchSeg* segs, int nsegs, int nitems, unsigned* ctr, unsigned long long spin_limit) {
-  cfk::batch_kernel(segs, nsegs, nitems, ctr, spin_limit);
+// 512 threads and about 76 KiB of LDS: two work-groups per CU, so at most 128 VGPRs
+extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(
+    const CfBatchSeg* segs, const int* stage_need, int nsegs, int nitems, unsigned* ctr,
+    unsigned long long spin_limit) {
+  cfk::batch_kernel<true>(segs, stage_need, nsegs, nitems, ctr, spin_limit);
 }
-
-// Zeroes the queue head, error flag and completion counters (a kernel rather
-// than a memset so that it can sit in a captured graph).
-extern "C" __global__ __launch_bounds__(256) void cf_batch_reset(unsigned* ctr, int n) {
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) ctr[i] = 0u;
+// the same for batches on the three-pass set only (plane plan at 32 and 64,
+// four-step g24m2h at 128 and 256), its tiles inlined
+extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run_3p(
+    const CfBatchSeg* segs, const int* stage_need, int nsegs, int nitems, unsigned* ctr,
+    unsigned long long spin_limit) {
+  cfk::batch_kernel<false>(segs, stage_need, nsegs, nitems, ctr, spin_limit);
 }
 
 // ---- FP64 table generation -------------------------------------------------

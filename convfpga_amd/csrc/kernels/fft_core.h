@@ -153,12 +153,27 @@ struct ShapeInfo {
   static_assert(Shape<N>::R0 * Shape<N>::R1 * Shape<N>::R2 == N, "radix plan must multiply to N");
 };
 
+// Orders the LDS accesses of the work-group (WAVE = false) or only of the
+// calling wave (WAVE = true: a line held by lanes of one wave, in LDS that no
+// other wave touches meanwhile -- a wave's LDS operations execute in order,
+// so a compiler fence at wavefront scope suffices).
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham radix-R step on the P registers of thread t.  NSP is the
 // product of the radices of the steps already done.  Virtual thread vv (one
 // R-point butterfly) reads points vv + r*N/R, rotates them by W_{NSP*R}^(k*r)
 // with k = vv mod NSP, does a DFT_R and writes point (vv-k)*R + k + r*NSP.
 // Non-final steps write through LDS and read back the t + T*m pattern.
-template <int N, int R, int NSP, bool LAST, bool INV, class Lds>
+template <int N, int R, int NSP, bool LAST, bool INV, bool WAVE, class Lds>
 __device__ __forceinline__ void fft_step(float2 (&v)[Shape<N>::P], int t, const Lds& lds, const float2* tw) {
   constexpr int P = Shape<N>::P, T = N / P, Q = P / R;
 #pragma unroll
@@ -183,21 +198,23 @@ __device__ __forceinline__ void fft_step(float2 (&v)[Shape<N>::P], int t, const 
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = lds.ld(t + T * m);
-    __syncthreads();
+    lds_sync<WAVE>();
   }
 }
 
 // Full length-N transform of the registers of thread t (see file comment).
-template <int N, bool INV, class Lds>
+// WAVE: the line's T threads are lanes of one wave and its LDS segment is
+// private to that wave (the caller orders any later reuse of it by others).
+template <int N, bool INV, bool WAVE = false, class Lds>
 __device__ __forceinline__ void fft_line(float2 (&v)[Shape<N>::P], int t, const Lds& lds, const float2* tw) {
   using S = Shape<N>;
   constexpr int NS = ShapeInfo<N>::NSTEPS;
-  fft_step<N, S::R0, 1, NS == 1, INV>(v, t, lds, tw);
-  if constexpr (NS >= 2) fft_step<N, S::R1, S::R0, NS == 2, INV>(v, t, lds, tw);
-  if constexpr (NS >= 3) fft_step<N, S::R2, S::R0 * S::R1, true, INV>(v, t, lds, tw);
+  fft_step<N, S::R0, 1, NS == 1, INV, WAVE>(v, t, lds, tw);
+  if constexpr (NS >= 2) fft_step<N, S::R1, S::R0, NS == 2, INV, WAVE>(v, t, lds, tw);
+  if constexpr (NS >= 3) fft_step<N, S::R2, S::R0 * S::R1, true, INV, WAVE>(v, t, lds, tw);
 }
 
 // LDS views.  Column layout (lines are adjacent x-columns, C per tile):

@@ -79,7 +79,8 @@ struct PlanePass {
   __device__ __forceinline__ void process(int tile, Regs& r) const {
     float2* dst = out + (size_t)tile * N * N;
     if constexpr (!INV) {
-      fft_line<N, false>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);
+      fft_line<N, false, true>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);  // a row: lanes of one wave
+      __syncthreads();  // every wave has read its rows before the transpose overwrites them
 #pragma unroll
       for (int m = 0; m < P; ++m) lds[ry * (N + 1) + rt + T * m] = r.v[m];  // rows -> columns
       __syncthreads();
@@ -97,7 +98,7 @@ struct PlanePass {
 #pragma unroll
       for (int m = 0; m < P; ++m) r.v[m] = lds[ry * (N + 1) + rt + T * m];
       __syncthreads();
-      fft_line<N, true>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);
+      fft_line<N, true, true>(r.v, rt, RowLds{lds + ry * K::ROW_PITCH}, tw_s);
 #pragma unroll
       for (int m = 0; m < P; ++m) dst[ry * N + rt + T * m] = cscale(r.v[m], scale);
     }
@@ -119,7 +120,7 @@ __device__ __forceinline__ void plane_kernel(const float2* in, float2* out, cons
 // y2 in [0, Y2 = N/16); every line (x row or z line) is held by T = N/16
 // threads of 16 points
 // ---------------------------------------------------------------------------
-template <int N_, int GA, int GB>
+template <int N_, int GA, int GB, int YB = 1>
 struct SplitGeo {
   static constexpr int N = N_;
   static constexpr int Y2 = N / 16;               // y2 values: the middle pass's DFT size
@@ -133,18 +134,23 @@ struct SplitGeo {
   // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
   static constexpr int A_COL_PLANE = N * 17 + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
-  // intermediate index of (x, k1, y2, z)
+  // intermediate index of (x, k1, y2, z): [k1][x/GB][z][y2/YB][x%GB][y2%YB].
+  // YB = 1: runs of GB x.  YB = 2: pairs of y2 interleaved into each run of
+  // GB x, so that an outer tile of 2 y2 values reads runs of 2*GB points while
+  // a middle tile of 2 x writes runs of 4.
   __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
-    return (((k1 * NXH + x / GB) * N + z) * Y2 + y2) * GB + (x % GB);
+    return ((((k1 * NXH + x / GB) * N + z) * (Y2 / YB) + y2 / YB) * GB + (x % GB)) * YB + y2 % YB;
   }
 };
 template <int GA, int GB>
 using Split256 = SplitGeo<256, GA, GB>;
 
-// INV = false: pass A (natural -> intermediate).  INV = true: pass C.
-template <int N, int GA, int GB, bool INV>
+// INV = false: pass A (natural -> intermediate).  INV = true: pass C (which
+// may read an intermediate with interleaved y2 pairs, YB = 2 = GA).
+template <int N, int GA, int GB, bool INV, int YB = 1>
 struct Xy1Pass {
-  using K = SplitGeo<N, GA, GB>;
+  using K = SplitGeo<N, GA, GB, YB>;
+  static_assert(YB == 1 || (INV && YB == GA), "interleaved y2 pairs: read by the inverse pass of y2 pairs");
   static constexpr int Y2 = K::Y2, T = K::T;
   static constexpr int K1_STRIDE = K::NXH * N * Y2 * GB;  // mid_index step of k1
   struct Regs {
@@ -187,21 +193,46 @@ struct Xy1Pass {
       for (int k1 = 0; k1 < 16; ++k1) r.v[k1] = buffer_ld2(rs, voff, (unsigned)(k1 * K1_STRIDE * sizeof(float2)));
     }
   }
-  __device__ __forceinline__ void process(int tile, Regs& r) const {
+  // The pass in three phases separated by work-group barriers.
+  //   forward: 1 x FFT of the row (wave-local exchanges); 2 rows -> columns
+  //            in LDS; 3 DFT-16 over y1, twiddle, store the intermediate
+  //   inverse: 1 twiddle, inverse DFT-16 over k1, columns -> LDS; 2 LDS ->
+  //            rows; 3 inverse x FFT of the row, scale, store
+  __device__ __forceinline__ void phase1(int tile, Regs& r) const {
     size_t cube;
     int z, y2_0;
     tile_pos(tile, cube, z, y2_0);
-    const int cy2 = y2_0 + cy2l;
-    float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;  // this thread's 16 y1 (or k1) slots
     if constexpr (!INV) {
-      fft_line<N, false>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
+      fft_line<N, false, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);  // a row: lanes of one wave
+    } else {
+      const int cy2 = y2_0 + cy2l;
+#pragma unroll
+      for (int k1 = 1; k1 < 16; ++k1) r.v[k1] = cmulw<true>(r.v[k1], tw_s[cy2 * k1]);
+      DFT<16, true>::run(r.v);  // r.v[y1]
+      float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;  // this thread's 16 k1 / y1 slots
+#pragma unroll
+      for (int j = 0; j < 16; ++j) colp[j] = r.v[j];
+    }
+  }
+  __device__ __forceinline__ void phase2(int, Regs& r) const {
+    if constexpr (!INV) {
 #pragma unroll
       for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1] = r.v[m];
-      __syncthreads();
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1];
+    }
+  }
+  __device__ __forceinline__ void phase3(int tile, Regs& r) const {
+    size_t cube;
+    int z, y2_0;
+    tile_pos(tile, cube, z, y2_0);
+    if constexpr (!INV) {
+      const int cy2 = y2_0 + cy2l;
+      const float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;
       float2 u[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) u[j] = colp[j];
-      __syncthreads();  // LDS free for the next tile
       DFT<16, false>::run(u);  // u[k1]
 #pragma unroll
       for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulw<false>(u[k1], tw_s[cy2 * k1]);
@@ -209,16 +240,7 @@ struct Xy1Pass {
 #pragma unroll
       for (int k1 = 0; k1 < 16; ++k1) dst[k1 * K1_STRIDE] = u[k1];
     } else {
-#pragma unroll
-      for (int k1 = 1; k1 < 16; ++k1) r.v[k1] = cmulw<true>(r.v[k1], tw_s[cy2 * k1]);
-      DFT<16, true>::run(r.v);  // r.v[y1]
-#pragma unroll
-      for (int j = 0; j < 16; ++j) colp[j] = r.v[j];
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1];
-      __syncthreads();
-      fft_line<N, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
+      fft_line<N, true, true>(r.v, rt, RowLds{lds + row * K::A_ROW_PITCH}, tw_s);
       if (nt) {  // the output is not read again by this convolution either
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(out + cube, (unsigned)(N * N * N * sizeof(float2)));
         const unsigned voff = (unsigned)(((z * N + Y2 * ry1 + y2_0 + ry2l) * N + rt) * sizeof(float2));
@@ -230,6 +252,14 @@ struct Xy1Pass {
         for (int m = 0; m < 16; ++m) dst[T * m] = cscale(r.v[m], scale);
       }
     }
+  }
+  // One tile by one set of threads (the phases and their barriers in order).
+  __device__ __forceinline__ void process(int tile, Regs& r) const {
+    phase1(tile, r);
+    __syncthreads();  // rows read (forward) / columns written (inverse) by every wave
+    phase2(tile, r);
+    __syncthreads();
+    phase3(tile, r);
   }
 };
 
@@ -249,25 +279,29 @@ __device__ __forceinline__ int xcd_group_tile(int b) {
   }
 }
 
-template <int N, int GA, int GB, bool INV>
+template <int N, int GA, int GB, bool INV, int YB = 1>
 __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
-  using K = SplitGeo<N, GA, GB>;
+  using K = SplitGeo<N, GA, GB, YB>;
+  using P = Xy1Pass<N, GA, GB, INV, YB>;
   constexpr int S = (GA * GB >= 16) ? 1 : 16 / (GA * GB);
   static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
   __shared__ float2 lds[K::A_LDS];
   __shared__ float2 tw_s[N];
   const int x = threadIdx.x;
   const int row = x / K::T;
-  const int cxl = x % GB, cy2l = (x / GB) % GA, cxh = x / (GB * GA);
+  // column identity in the intermediate's address order (consecutive lanes,
+  // consecutive addresses): x%GB then y2l, or y2l then x%GB (YB = 2)
+  const int cxl = YB == 1 ? x % GB : (x / YB) % GB;
+  const int cy2l = YB == 1 ? (x / GB) % GA : x % YB;
+  const int cxh = x / (GB * GA);
   // A single convolution keeps its intermediate in the MALL better when the
   // signal and spectrum, read once, are loaded non-temporally (measured at
   // 256^3: 178 against 186 us per convolution); a launch over several cubes
   // is about 1% slower that way, so it keeps ordinary loads.
   const bool nt = ntotal <= K::A_TILES;
-  const Xy1Pass<N, GA, GB, INV> k{in, out, tw_s, lds, scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl,
-                                  cy2l, nt};
+  const P k{in, out, tw_s, lds, scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, nt};
   const int tile = xcd_group_tile<S>((int)blockIdx.x);
-  typename Xy1Pass<N, GA, GB, INV>::Regs a;
+  typename P::Regs a;
   // At 256^3 the forward pass issues its loads before staging the twiddles
   // (the two latencies overlap: 54.3 -> 54.0 us, with the middle pass below
   // 1.3% per convolution); the inverse pass, whose first step multiplies by
@@ -309,12 +343,13 @@ using MidGeo = MidGeoN<256, GB, GM>;
 // place): the tile's output is one contiguous block of N*Y2*GM points, so it
 // stores whole lines however narrow the tile (the reads stay short runs,
 // shared with the sibling tiles through L2).
-template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB>
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1>
 struct MidPass {
   using K = SplitGeo<N, GA, GB>;
-  using KO = SplitGeo<N, GA, GBO>;
+  using KO = SplitGeo<N, GA, GBO, YBO>;
   using M = MidGeoN<N, GB, GM>;
-  static_assert(GBO == GB || GBO == GM, "the output layout is the input's or the tile's own block");
+  static_assert((YBO == 1 && (GBO == GB || GBO == GM)) || (YBO == 2 && GBO == GB),
+                "the output layout is the input's, the tile's own block, or the input's with y2 pairs interleaved");
   static constexpr int L = M::LINES, Y2 = M::Y2, T = M::T, RPT = M::RPT;
   struct Regs {
     float2 v[16];
@@ -441,11 +476,11 @@ struct MidPass {
   }
 };
 
-template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB>
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
   using M = MidGeoN<N, GB, GM>;
-  using P = MidPass<N, GA, GB, GM, HPF, GBO>;
+  using P = MidPass<N, GA, GB, GM, HPF, GBO, YBO>;
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[N];

@@ -37,7 +37,7 @@ extern "C" {
                                * accumulation, one power-of-two scale per tile.
                                * N = 256 with a spectrum of a four-step plan
                                * whose middle tiles are 2 x wide (split3_g22,
-                               * g42, g24m2, g28m2, g24m2o, g28m2o); other
+                               * g42, g24m2, g28m2 and their o / h forms); other
                                * spectra: CF_ERR_ARG */
 
 /* execution plans (which passes and tiles a convolution of edge N runs) */
@@ -65,7 +65,12 @@ extern "C" {
  * buffer holds the first one.  Spectrum layout of the other GM = 2 plans. */
 #define CF_PLAN_SPLIT3_G24M2O 13
 #define CF_PLAN_SPLIT3_G28M2O 14
-#define CF_NUM_PLANS 15
+/* the same, with the second intermediate in the first one's x groups and
+ * pairs of y2 interleaved: the middle pass writes runs of 4 points, the last
+ * pass reads runs of 8 (G24M2H) or 16 (G28M2H) */
+#define CF_PLAN_SPLIT3_G24M2H 15
+#define CF_PLAN_SPLIT3_G28M2H 16
+#define CF_NUM_PLANS 17
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
@@ -156,11 +161,13 @@ int cf_conv3d_share_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsig
                           void *stream, cf_pass_times_t *times);
 
 /* Event profiling of the next cf_conv3d_device calls without a host sync per
- * call: cf_profile_begin arms it for up to max_calls calls, cf_profile_end
- * waits for them and returns each pass's mean device time over the recorded
- * calls (times->ms, from events recorded on the launch stream between the
- * passes) and the number of calls recorded. */
-int cf_profile_begin(unsigned max_calls);
+ * call: cf_profile_begin arms it for up to max_calls recorded calls, taking
+ * every `every`-th call (1: each one; the others run without events, whose
+ * packets between the passes cost the GPU a few microseconds each);
+ * cf_profile_end waits for them and returns each pass's mean device time over
+ * the recorded calls (times->ms, from events recorded on the launch stream
+ * between the passes) and the number of calls recorded. */
+int cf_profile_begin(unsigned max_calls, unsigned every);
 int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
 
 /* ---- the same without any allocation ---------------------------------------
@@ -187,9 +194,12 @@ int cf_conv3d_device_noalloc(cf_spectrum_t h, const void *d_sig, void *d_out, un
  * of count 1 -- instances may share a spectrum) prepared once and replayed
  * with one graph launch per call.
  *   CF_BATCH_PERSISTENT: one persistent kernel runs every pass tile of every
- *     instance from a single work queue (pass-major order, per-instance pass
- *     dependencies); edges 16..256, spectra of the natural-order plans
- *     (CF_PLAN_AXIS5 / CF_PLAN_PLANE3).
+ *     instance from a single work queue, stage by stage (all first passes,
+ *     then all second passes, ...), each instance on the plan of its spectrum:
+ *     CF_PLAN_AXIS5 (N = 16..256), CF_PLAN_PLANE3 (N = 32, 64),
+ *     CF_PLAN_SPLIT3_G24M2 or CF_PLAN_SPLIT3_G24M2H (N = 128, 256); other plans:
+ *     CF_ERR_ARG.  The batch holds the intermediates of its four-step
+ *     instances.
  *   CF_BATCH_GRAPH: the passes of each instance's own plan, recorded in order
  *     into one graph (any edge and plan).
  * An output may alias its own signal.  In CF_BATCH_PERSISTENT mode the
@@ -211,6 +221,13 @@ int cf_batch_launch(cf_batch_t b, void *stream);
 /* waits for the last launch; CF_ERR_RUNTIME if the persistent kernel reported a stalled dependency */
 int cf_batch_check(cf_batch_t b);
 void cf_batch_destroy(cf_batch_t b);
+/* Test support for the persistent kernel's bounded wait: a CF_BATCH_PERSISTENT
+ * batch whose last stage waits for one tile more than the stage before it
+ * completes, polling at most spin_limit times; every launch therefore ends
+ * with the error cf_batch_check reports, and with the batch's last pass not
+ * run (its outputs are incomplete).  For tests of the error path only. */
+int cf_batch_create_stall_test(const cf_instance_t *inst, unsigned count, int scale_mode,
+                               unsigned long long spin_limit, cf_batch_t *out);
 
 /* ---- plain 3D transforms (device) ------------------------------------------ */
 /* forward (inverse = 0) or unnormalised inverse (inverse = 1) 3D FFT */

@@ -290,7 +290,7 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
     [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)] + \
-    [(n, p) for n in (128, 256) for p in (13, 14)]
+    [(n, p) for n in (128, 256) for p in (13, 14, 15, 16)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
@@ -332,7 +332,7 @@ def test_every_plan_matches_oracle(gpu, oracle, n, plan):
         api.set_plan(n, api.CF_PLAN_AUTO)
 
 
-@pytest.mark.parametrize("plan", [1, 3, 6, 12, 13])
+@pytest.mark.parametrize("plan", [1, 3, 6, 12, 13, 16])
 def test_dropin_256_on_each_plan(gpu, oracle, plan):
     n = 256
     print("case drop-in N=256 plan=%d" % plan, flush=True)
@@ -374,7 +374,7 @@ def _spectrum_order(api, n, plan, spec64):
         return spec64
     if name == "axis5z":
         return np.ascontiguousarray(spec64.reshape(n, n, n // 16, 16).transpose(2, 1, 0, 3)).reshape(-1)
-    tail = name.split("_g")[1].rstrip("o")  # <GA><GB>[m<GM>][o]
+    tail = name.split("_g")[1].rstrip("oh")  # <GA><GB>[m<GM>][o|h]
     gm = int(tail.split("m")[1]) if "m" in tail else int(tail[1])
     ref = spec64.reshape(n, n // 16, 16, n // gm, gm)  # (kz, k2, k1, kx/GM, kx%GM)
     return np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
@@ -482,6 +482,44 @@ def test_batch_mixed_sizes_replayed(gpu, oracle, mode, plan256):
         x = torch.ones(1 << 20, device="cuda")
         assert float((x * 2).sum()) == 2.0 * (1 << 20)
     b.close()
+
+
+@pytest.mark.parametrize("n,plan,count", [(64, 2, 3), (128, 7, 2), (128, 15, 2), (256, 7, 1), (256, 15, 1),
+                                          (32, 2, 4)])
+def test_persistent_batch_on_each_plan(gpu, oracle, n, plan, count):
+    """The persistent kernel on each plan it runs (plane, four-step g24m2 and
+    g24m2h): every instance against the FP64 oracle and bitwise against the
+    same spectra in graph mode (the same pass code), over two replays."""
+    import torch
+    api = gpu.api
+    P = n ** 3
+    print("case persistent batch N=%d plan=%s count=%d" % (n, api.PLAN_NAMES[plan], count), flush=True)
+    sig = [oracle.uniform(P, 85000 + 11 * n + k) for k in range(count)]
+    flt = [oracle.uniform(P, 86000 + 11 * n + k) for k in range(count)]
+    api.set_plan(n, plan)
+    try:
+        specs = [gpu.Spectrum(n, _t(f)) for f in flt]
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    ds = [_t(x) for x in sig]
+    op = [torch.zeros_like(d) for d in ds]
+    og = [torch.zeros_like(d) for d in ds]
+    bp = api.Batch(list(zip(ds, op, specs)), mode=api.CF_BATCH_PERSISTENT)
+    bg = api.Batch(list(zip(ds, og, specs)), mode=api.CF_BATCH_GRAPH)
+    for rep in range(2):
+        for o in op + og:
+            o.zero_()
+        bp.launch()
+        bp.check()
+        bg.launch()
+        bg.check()
+        for k in range(count):
+            assert torch.equal(op[k].view(torch.int64), og[k].view(torch.int64)), (rep, k)
+    for k in range(count):
+        ref = oracle.conv3d_f64(n, sig[k], flt[k], 1.0 / P)
+        assert oracle.rel_l2(_np(op[k]), ref) <= REL_L2_FP32, k
+    bp.close()
+    bg.close()
 
 
 def test_spectrum_views_outlive_a_closed_parent(gpu, oracle):
@@ -684,7 +722,7 @@ def test_batch_shared_spectrum_in_place(gpu, oracle):
     b.close()
 
 
-def test_batch_stalled_dependency_is_reported(gpu, oracle, monkeypatch):
+def test_batch_stalled_dependency_is_reported(gpu, oracle):
     """The persistent kernel's bounded wait: a dependency made unsatisfiable
     from the host (with a short spin limit) ends the launch with an error
     instead of a hang; a batch built afterwards runs normally."""
@@ -693,9 +731,7 @@ def test_batch_stalled_dependency_is_reported(gpu, oracle, monkeypatch):
     f = oracle.uniform(n ** 3, 71001)
     spec = gpu.Spectrum(n, _t(f))
     d = _t(oracle.uniform(n ** 3, 71002))
-    monkeypatch.setenv("CONVFPGA_TEST_BATCH_STALL", "1")
-    b = gpu.api.Batch([(d, d.clone(), spec)])
-    monkeypatch.delenv("CONVFPGA_TEST_BATCH_STALL")
+    b = gpu.api.Batch([(d, d.clone(), spec)], stall_test_spin_limit=4096)
     b.launch()
     with pytest.raises(gpu.ConvError, match="dependency"):
         b.check()
@@ -707,10 +743,10 @@ def test_batch_stalled_dependency_is_reported(gpu, oracle, monkeypatch):
     ok.close()
 
 
-@pytest.mark.parametrize("plan,what", [(4, "four-step"), (12, "z-blocked")])
-def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle, plan, what):
-    """The persistent batch reads spectra in natural order: spectra in the
-    private layouts of the four-step and z-blocked plans are refused."""
+@pytest.mark.parametrize("plan", [4, 12])
+def test_batch_rejects_other_plans_in_persistent_mode(gpu, oracle, plan):
+    """The persistent batch runs the axis, plane and two four-step plans:
+    spectra of the others (here g24 and the z-blocked plan) are refused."""
     api = gpu.api
     api.set_plan(256, plan)
     try:
@@ -718,7 +754,7 @@ def test_batch_rejects_four_step_spectrum_in_persistent_mode(gpu, oracle, plan, 
     finally:
         api.set_plan(256, api.CF_PLAN_AUTO)
     d = _t(oracle.uniform(256 ** 3, 72002))
-    with pytest.raises(gpu.ConvError, match=what):
+    with pytest.raises(gpu.ConvError, match="persistent batch runs"):
         api.Batch([(d, d, spec)], mode=api.CF_BATCH_PERSISTENT)
 
 
