@@ -42,7 +42,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
-PROFILE_EVERY = 4  # steps of the timed loop that carry per-pass events: one in PROFILE_EVERY
+PROFILE_EVERY = 20  # steps of the timed loop that carry per-pass events: one in PROFILE_EVERY
 
 SINGLE = {"c2_256": 256, "c2_128": 128, "c1_64": 64, "c5_512": 512}
 SPLIT = {"c5_split": [256] * 4 + [512]}

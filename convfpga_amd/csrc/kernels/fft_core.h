@@ -4,7 +4,8 @@
 // radix-2^2 feed-forward pipeline streaming 8 points per cycle) with a
 // Stockham autosort formulation: each thread holds P points of a line in
 // registers, does radix-R DFTs on them, and lines are re-distributed between
-// radix steps through LDS.  Output is in natural order, so nothing like the
+// radix steps through LDS (at N = 256, for a line held by one 16-lane DPP row,
+// through lane permutations instead).  Output is in natural order, so nothing like the
 // reference's bit-reversal buffers (kernels/matrixTranspose/diagonal_bitrev.cl)
 // is needed.
 //
@@ -205,13 +206,94 @@ __device__ __forceinline__ void fft_step(float2 (&v)[Shape<N>::P], int t, const 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register exchange for N = 256 (two radix-16 steps, 16 threads per line, a
+// line per 16-lane DPP row).  Between the steps the line's 16 x 16 points are
+// transposed across its threads: thread t's a[r] becomes thread r's former
+// a[t].  Done with lane permutations instead of an LDS round trip, in four
+// stages that each swap the off-diagonal halves for one bit of (thread,
+// register):
+//   lane bits 3, 2: row rotations (DPP row_ror) whose bank mask (groups of 4
+//                   lanes) selects the receiving half -- one move per value;
+//   lane bits 1, 0: quad permutations and a select -- two moves and a select.
+// 192 VALU moves per thread in place of 16 LDS stores, 16 LDS loads and a
+// wave barrier.
+// ---------------------------------------------------------------------------
+template <int CTRL, int BANKS>
+__device__ __forceinline__ float dpp_upd(float old, float src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, src), CTRL, 0xF, BANKS, false));
+}
+template <int CTRL, int BANKS>
+__device__ __forceinline__ float2 dpp_upd2(float2 old, float2 src) {
+  return make_float2(dpp_upd<CTRL, BANKS>(old.x, src.x), dpp_upd<CTRL, BANKS>(old.y, src.y));
+}
+template <int CTRL>
+__device__ __forceinline__ float2 dpp_mov2(float2 src) {
+  return make_float2(
+      __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src.x), CTRL, 0xF, 0xF, false)),
+      __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, src.y), CTRL, 0xF, 0xF, false)));
+}
+// DPP controls: row_ror:n = 0x120 + n (lane i of a row reads lane i - n mod 16);
+// quad_perm [1,0,3,2] = 0xB1 (lane ^ 1), [2,3,0,1] = 0x4E (lane ^ 2)
+constexpr int kRowRor4 = 0x124, kRowRor8 = 0x128, kRowRor12 = 0x12C, kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;
+
+// swap for lane bit 2 (B = 0) or 3 (B = 1) of registers i and i + S
+template <int B, int S>
+__device__ __forceinline__ void xpose_bank_stage(float2 (&a)[16]) {
+  constexpr int UP = B == 0 ? kRowRor4 : kRowRor8;     // lanes with the bit set read lane - 4 / - 8
+  constexpr int DOWN = B == 0 ? kRowRor12 : kRowRor8;  // lanes with the bit clear read lane + 4 / + 8
+  constexpr int SET = B == 0 ? 0xA : 0xC, CLEAR = B == 0 ? 0x5 : 0x3;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (!(i & S)) {
+      const float2 lo = a[i], hi = a[i + S];
+      a[i] = dpp_upd2<UP, SET>(lo, hi);
+      a[i + S] = dpp_upd2<DOWN, CLEAR>(hi, lo);
+    }
+}
+// swap for lane bit 0 (CTRL = kQuadXor1) or 1 (kQuadXor2); `set`: this lane's bit
+template <int CTRL, int S>
+__device__ __forceinline__ void xpose_quad_stage(float2 (&a)[16], bool set) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (!(i & S)) {
+      const float2 x = dpp_mov2<CTRL>(a[i + S]), y = dpp_mov2<CTRL>(a[i]);
+      a[i] = set ? x : a[i];
+      a[i + S] = set ? a[i + S] : y;
+    }
+}
+__device__ __forceinline__ void xpose16_rows(float2 (&a)[16], int t) {
+  xpose_bank_stage<1, 8>(a);
+  xpose_bank_stage<0, 4>(a);
+  xpose_quad_stage<kQuadXor2, 2>(a, t & 2);
+  xpose_quad_stage<kQuadXor1, 1>(a, t & 1);
+}
+// Length-256 transform in registers (no LDS but the twiddle table) of a line
+// whose 16 threads are lanes 16j .. 16j+15 (t = lane % 16): thread t holds
+// points t + 16 m before and after.
+template <bool INV>
+__device__ __forceinline__ void fft256_reg(float2 (&v)[16], int t, const float2* tw) {
+  DFT<16, INV>::run(v);  // point 16 t + r
+  xpose16_rows(v, t);    // point t + 16 m
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = cmulw<INV>(v[r], tw[r * t]);
+  DFT<16, INV>::run(v);
+}
+
 // Full length-N transform of the registers of thread t (see file comment).
 // WAVE: the line's T threads are lanes of one wave and its LDS segment is
-// private to that wave (the caller orders any later reuse of it by others).
+// private to that wave (the caller orders any later reuse of it by others);
+// at N = 256 they are lanes 16j .. 16j+15 (t = lane % 16) and the exchange
+// is done in registers, without the LDS segment.
 template <int N, bool INV, bool WAVE = false, class Lds>
 __device__ __forceinline__ void fft_line(float2 (&v)[Shape<N>::P], int t, const Lds& lds, const float2* tw) {
   using S = Shape<N>;
   constexpr int NS = ShapeInfo<N>::NSTEPS;
+  if constexpr (N == 256 && WAVE) {
+    fft256_reg<INV>(v, t, tw);
+    return;
+  }
   fft_step<N, S::R0, 1, NS == 1, INV, WAVE>(v, t, lds, tw);
   if constexpr (NS >= 2) fft_step<N, S::R1, S::R0, NS == 2, INV, WAVE>(v, t, lds, tw);
   if constexpr (NS >= 3) fft_step<N, S::R2, S::R0 * S::R1, true, INV, WAVE>(v, t, lds, tw);
