@@ -604,9 +604,8 @@ def run_batch(a, torch, C, api):
 def run_split(a, torch, C, api):
     """4 x 256^3 + 1 x 512^3 convolutions with their own filters, for U[-1,1)
     and for log-uniform inputs (both signal and filter): the split-FP16 MFMA
-    form beside the FP32 form.  The MFMA form exists for the 256^3 four-step
-    kernels; the 512^3 instance runs the FP32 five-pass schedule of the
-    autotuned plan (the z-blocked one on MI355X) in both columns.
+    form beside the FP32 form, at both edges (256^3: the four-step passes;
+    512^3: the five axis passes of the autotuned plan, natural or z-blocked).
     A step is the whole batch (one call for the four 256^3 cubes, one for the
     512^3 cube), inputs resident in HBM, filter spectra precomputed."""
     n1, b1, n2 = 256, 4, 512
@@ -637,7 +636,7 @@ def run_split(a, torch, C, api):
                 C.conv3d_device(spec1, sig1, out1, how_many=b1, precision=prec)
 
             def part2():
-                C.conv3d_device(spec2, sig2, out2)
+                C.conv3d_device(spec2, sig2, out2, precision=prec)
 
             def timed(fn, steps):
                 for _ in range(a.warmup):
@@ -660,6 +659,7 @@ def run_split(a, torch, C, api):
             host["out256_" + pname] = out1.cpu().numpy()
             host["out512_" + pname] = out2.cpu().numpy()
         entry["split_vs_fp32_speed_256"] = entry["fp32"]["ms_4x256"] / entry["split_fp16_mfma"]["ms_4x256"]
+        entry["split_vs_fp32_speed_512"] = entry["fp32"]["ms_512"] / entry["split_fp16_mfma"]["ms_512"]
         dists[dist] = entry
         checks[dist] = host
         del spec1, spec2, sig1, flt1, sig2, flt2, out1, out2
@@ -673,12 +673,12 @@ def run_split(a, torch, C, api):
                      "conv/s", u["ms_per_batch"],
                      {"workload": "c5_split_4x256^3+1x512^3", "instances": b1 + 1, "filters": "own per instance",
                       "scale": "1/N^3", "plans": {"256": "split3_g24m2", "512": plan512},
-                      "split_fp16_sizes": [256]})
-    line["dtype"] = "split-fp16 mfma (fp32 accumulate) for 256^3; fp32 for 512^3"
+                      "split_fp16_sizes": [256, 512]})
+    line["dtype"] = "split-fp16 mfma (fp32 accumulate)"
     line["data"] = "synthetic: U[-1,1) and log-uniform magnitudes 1e-3..1e3 with uniform phase, generated on the GPU"
     line["by_distribution"] = {d: {k: ({kk: round(vv, 5) for kk, vv in v.items()} if isinstance(v, dict) else round(v, 4))
                                    for k, v in e.items()} for d, e in dists.items()}
-    line["roofline"] = {"bound": "hbm", "kernel": "whole batch (split-FP16 256^3 + FP32 512^3)",
+    line["roofline"] = {"bound": "hbm", "kernel": "whole batch (split-FP16 256^3 + split-FP16 512^3)",
                         "achieved": alg / (u["ms_per_batch"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": alg / (u["ms_per_batch"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                         "bytes_model": "4 x 7V (256^3 four-step) + 11V (512^3 five-pass)"}
@@ -705,7 +705,7 @@ def split_check_leg(checks, reps):
                 e[form].append(O.rel_l2(got, ref))
             del ref
         acc[dist] = {k: [float("%.3e" % x) for x in v] for k, v in e.items()}
-        acc[dist]["instances"] = ["256^3"] * 4 + ["512^3 (fp32 in both columns)"]
+        acc[dist]["instances"] = ["256^3"] * 4 + ["512^3"]
     base, _ = cpu_baseline_leg(256, None, 1, 2, reps)
     return base, {"rel_l2_vs_fp64": acc}
 

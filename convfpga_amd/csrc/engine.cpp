@@ -69,6 +69,9 @@ struct SizeFns {
   hipFunction_t xy1_fwd[cfgeo::kSplitVariants] = {}, xy1_inv[cfgeo::kSplitVariants] = {};  // N = 256
   hipFunction_t mid[cfgeo::kSplitVariants] = {}, mspec[cfgeo::kSplitVariants] = {};
   hipFunction_t mx_xy1_fwd = nullptr, mx_xy1_inv = nullptr, mx_mid = nullptr;  // N = 256, split-FP16 MFMA
+  // N = 512, split-FP16 MFMA axis passes: natural layout, then z-blocked (AXIS5Z)
+  hipFunction_t mx_row_fwd = nullptr, mx_row_inv = nullptr, mx_coly_fwd = nullptr, mx_coly_inv = nullptr;
+  hipFunction_t mx_colz_conv = nullptr, mx_coly_fwd_zb = nullptr, mx_colz_conv_zb = nullptr, mx_coly_inv_zb = nullptr;
   void* tw = nullptr;  // N float2 twiddles W_N^j
   int forced = CF_PLAN_AUTO;  // cf_set_plan
   int tuned = CF_PLAN_AUTO;   // autotuner's choice, CF_PLAN_AUTO until it has run
@@ -321,10 +324,20 @@ int size_res(unsigned n, SizeFns** out) {
     CFCHK(get_fn(&s.mx_xy1_inv, "cf_xy1_inv_256_s24"));
     CFCHK(get_fn(&s.mx_mid, "cf_mid_256_s24m2"));
   }
+  if (n == 512) {
+    CFCHK(get_fn(&s.mx_row_fwd, "cf_row_fwd_512_s"));
+    CFCHK(get_fn(&s.mx_row_inv, "cf_row_inv_512_s"));
+    CFCHK(get_fn(&s.mx_coly_fwd, "cf_coly_fwd_512_s"));
+    CFCHK(get_fn(&s.mx_coly_inv, "cf_coly_inv_512_s"));
+    CFCHK(get_fn(&s.mx_colz_conv, "cf_colz_conv_512_s"));
+    CFCHK(get_fn(&s.mx_coly_fwd_zb, "cf_coly_fwd_zb_512_s"));
+    CFCHK(get_fn(&s.mx_colz_conv_zb, "cf_colz_conv_zb_512_s"));
+    CFCHK(get_fn(&s.mx_coly_inv_zb, "cf_coly_inv_zb_512_s"));
+  }
   read_tune_file(n, &s);
-  // twiddles W_N^j; for N = 256 followed by the DFT16 operand table of the
-  // split-FP16 kernels (2 x 64 lanes x 4 x 16 bytes)
-  const size_t tw_bytes = (size_t)n * 8, table_bytes = n == 256 ? 8192 : 0;
+  // twiddles W_N^j; for N = 256 and 512 followed by the DFT16 operand table
+  // of the split-FP16 kernels (2 x 64 lanes x 4 x 16 bytes)
+  const size_t tw_bytes = (size_t)n * 8, table_bytes = n == 256 || n == 512 ? 8192 : 0;
   HIPCHK(hipMalloc(&s.tw, tw_bytes + table_bytes));
   void* tw = s.tw;
   int nn = (int)n;
@@ -420,6 +433,24 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
   const double V = (double)cube_bytes(n);
   const double io = 2.0 * V * nb;
   const double hbytes = hstride ? V * nb : V;
+  if (precision == CF_PREC_SPLIT_FP16 && n == 512) {  // AXIS5 or AXIS5Z spectrum (checked by the caller)
+    const unsigned rt = 256, rT = 512 * 512 / 8, ct = 512, cT = 32 * 512;
+    v.push_back({f.mx_row_fwd, rT, rt, sig, out, nullptr, 0, 1.0f, "x_fwd_mfma", io});
+    if (plan == CF_PLAN_AXIS5Z) {
+      v.push_back({f.mx_coly_fwd_zb, cT, ct, out, work, nullptr, 0, 1.0f, "y_fwd_mfma", io});
+      v.push_back({f.mx_colz_conv_zb, cT, ct, work, work, H, hstride, 1.0f, "z_mfma_mul_inv", io + hbytes});
+      const size_t split = v.size();
+      v.push_back({f.mx_coly_inv_zb, cT, ct, work, out, nullptr, 0, 1.0f, "y_inv_mfma", io});
+      v.push_back({f.mx_row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv_mfma", io});
+      return split;
+    }
+    v.push_back({f.mx_coly_fwd, cT, ct, out, out, nullptr, 0, 1.0f, "y_fwd_mfma", io});
+    v.push_back({f.mx_colz_conv, cT, ct, out, out, H, hstride, 1.0f, "z_mfma_mul_inv", io + hbytes});
+    const size_t split = v.size();
+    v.push_back({f.mx_coly_inv, cT, ct, out, out, nullptr, 0, 1.0f, "y_inv_mfma", io});
+    v.push_back({f.mx_row_inv, rT, rt, out, out, nullptr, 0, scale, "x_inv_mfma", io});
+    return split;
+  }
   if (precision == CF_PREC_SPLIT_FP16) {  // four-step plan with middle tiles of 2 x (checked by the caller)
     const unsigned at = 512, aT = 256 * 8, bt = 512, bT = 16 * 128;
     v.push_back({f.mx_xy1_fwd, aT, at, sig, work, nullptr, 0, 1.0f, "xy1_fwd_mfma", io});
@@ -1377,10 +1408,12 @@ static int check_conv_args(cf_spectrum_t h, unsigned how_many, int scale_mode, i
   if (precision != CF_PREC_FP32 && precision != CF_PREC_SPLIT_FP16)
     return fail(CF_ERR_ARG, "unknown precision %d", precision);
   if (precision == CF_PREC_SPLIT_FP16 &&
-      (h->n != 256 || !is_split(h->plan) || cfgeo::split_gm(split_variant(h->plan)) != 2))
+      !(h->n == 256 && is_split(h->plan) && cfgeo::split_gm(split_variant(h->plan)) == 2) &&
+      !(h->n == 512 && (h->plan == CF_PLAN_AXIS5 || h->plan == CF_PLAN_AXIS5Z)))
     return fail(CF_ERR_ARG,
                 "split-FP16 precision runs N = 256 with a spectrum of a four-step plan with middle tiles of 2 x "
-                "(split3_g22, g42, g24m2, g28m2); this spectrum is N = %u, plan %d",
+                "(split3_g22, g42, g24m2, g28m2 and their o/h forms) and N = 512 with a spectrum of axis5 or "
+                "axis5z; this spectrum is N = %u, plan %d",
                 h->n, h->plan);
   return CF_OK;
 }

@@ -15,6 +15,7 @@
 #include "fft_core.h"
 #include "three_pass.h"
 #include "split_mfma.h"
+#include "split_mfma512.h"
 #include "../geometry.h"
 
 namespace cfk {
@@ -196,6 +197,27 @@ extern "C" __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
   cfk::mid_mfma_kernel<4, 2>(in, out, H, tw, hstride, ntotal);
 }
 static_assert(cfk::Xy1MfmaGeo<2, 4>::THREADS == 512 && cfk::MidMfmaGeo<4, 2>::THREADS == 512, "split-FP16 geometry");
+
+// ---- split-FP16 MFMA form of the 512^3 axis passes (split_mfma512.h) ----------------
+// natural layout (spectra of CF_PLAN_AXIS5) and the z-blocked intermediate
+// (spectra of CF_PLAN_AXIS5Z); 2 work-groups of 512 threads per CU for the columns
+extern "C" __global__ __launch_bounds__(256) void cf_row_fwd_512_s(CF_ARGS) {
+  cfk::row512_mfma_kernel<false>(in, out, tw, scale, ntotal);
+}
+extern "C" __global__ __launch_bounds__(256) void cf_row_inv_512_s(CF_ARGS) {
+  cfk::row512_mfma_kernel<true>(in, out, tw, scale, ntotal);
+}
+#define CF_COL512_S(NAME, AXIS, MODE, IZ, OZ)                                                                 \
+  extern "C" __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void NAME(CF_ARGS) {   \
+    cfk::col512_mfma_kernel<AXIS, MODE, IZ, OZ>(in, out, H, tw, hstride, ntotal);                             \
+  }
+CF_COL512_S(cf_coly_fwd_512_s, 1, 0, false, false)
+CF_COL512_S(cf_coly_inv_512_s, 1, 1, false, false)
+CF_COL512_S(cf_colz_conv_512_s, 2, 2, false, false)
+CF_COL512_S(cf_coly_fwd_zb_512_s, 1, 0, false, true)
+CF_COL512_S(cf_colz_conv_zb_512_s, 2, 2, true, true)
+CF_COL512_S(cf_coly_inv_zb_512_s, 1, 1, true, false)
+static_assert(cfk::Row512Mfma::THREADS == 256 && cfk::Col512Mfma::THREADS == 512, "split-FP16 512 geometry");
 
 // The A operands of the 16-point DFT products (hi and lo FP16 parts of W/16
 // and conj(W)/16, per lane) from double-precision sincospi: 128 threads.

@@ -37,8 +37,11 @@ extern "C" {
                                * accumulation, one power-of-two scale per tile.
                                * N = 256 with a spectrum of a four-step plan
                                * whose middle tiles are 2 x wide (split3_g22,
-                               * g42, g24m2, g28m2 and their o / h forms); other
-                               * spectra: CF_ERR_ARG */
+                               * g42, g24m2, g28m2 and their o / h forms), and
+                               * N = 512 with a spectrum of CF_PLAN_AXIS5 or
+                               * CF_PLAN_AXIS5Z (512 = 16 * 16 * 2: two matrix
+                               * stages and an FP32 radix-2 stage per axis);
+                               * other spectra: CF_ERR_ARG */
 
 /* execution plans (which passes and tiles a convolution of edge N runs) */
 #define CF_PLAN_AUTO 0       /* the autotuner's choice (default)                              */

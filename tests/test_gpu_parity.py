@@ -798,6 +798,40 @@ def test_split_fp16_256_matches_oracle(gpu, oracle, dist):
         api.set_plan(n, api.CF_PLAN_AUTO)
 
 
+@pytest.mark.parametrize("dist,plan", [("uniform", "AXIS5Z"), ("loguniform", "AXIS5")])
+def test_split_fp16_512_matches_oracle(gpu, oracle, dist, plan):
+    """Split-FP16 MFMA form of the 512^3 axis passes on the natural (axis5)
+    and the z-blocked (axis5z) spectrum layout: rel-L2 against the FP64
+    oracle within the stated bound, for U[-1,1) and for log-uniform inputs;
+    in place gives the same bits."""
+    import torch
+    n = 512
+    P = n ** 3
+    print("case split-FP16 N=512 %s plan %s" % (dist, plan), flush=True)
+    api = gpu.api
+    gen = oracle.uniform if dist == "uniform" else oracle.loguniform
+    sig, filt = gen(P, 18101), gen(P, 18102)
+    api.set_plan(n, getattr(api, "CF_PLAN_" + plan))
+    try:
+        spec = gpu.Spectrum(n, _t(filt))
+        ds = _t(sig)
+        d16 = torch.empty_like(ds)
+        gpu.conv3d_device(spec, ds, d16, precision=api.CF_PREC_SPLIT_FP16)
+        g16 = _np(d16)
+        del d16
+        ref = oracle.conv3d_f64(n, sig, filt, 1.0 / P)
+        e16 = oracle.rel_l2(g16, ref)
+        print("  rel-L2 split-FP16 %.3e" % e16, flush=True)
+        assert e16 <= REL_L2_SPLIT
+        del ref
+        gpu.conv3d_device(spec, ds, ds, precision=api.CF_PREC_SPLIT_FP16)
+        assert np.array_equal(_np(ds), g16)
+        del spec, ds
+        torch.cuda.empty_cache()
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+
+
 def test_split_fp16_rejects_other_layouts(gpu, oracle):
     import torch
     print("case split-FP16 argument checks", flush=True)
