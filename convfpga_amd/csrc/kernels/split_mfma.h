@@ -441,9 +441,10 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
     }
   const int e1 = fp16_exp(block_max<Q::WAVES>(absmax16(re, im), red[0]));
   scale_soa(re, im, ldexpf(1.0f, e1));
-  Dft16Ops ops = dft16_ops<false>(optab);
+  // (the operand table is re-read before each stage rather than held in 16
+  // VGPRs across the pass: the kernel stays within 128 VGPRs without spills)
   // z: stage A over m for t = 4zq+b -> k0 = 4g+i, twiddle W256^(t k0)
-  dft16x4_mfma(re, im, ops);
+  dft16x4_mfma(re, im, dft16_ops<false>(optab));
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -458,7 +459,7 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
     for (int i = 0; i < 4; ++i)
       lds_get(lds, (4 * g + i) * Q::E_ZT + (4 * zq + b) * Q::E_K0 + line, re[4 * b + i], im[4 * b + i]);
   // stage B over t for k0 = 4zq+b -> k1' = 4g+i: kz = k0 + 16 k1'
-  dft16x4_mfma(re, im, ops);
+  dft16x4_mfma(re, im, dft16_ops<false>(optab));
   __syncthreads();  // E read everywhere before F overwrites it
 #pragma unroll
   for (int b = 0; b < 4; ++b)
@@ -473,7 +474,7 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
 #pragma unroll
     for (int i = 0; i < 4; ++i) lds_get(lds, kz * Q::F_KZ + (4 * g + i) * GM + xb, re[4 * b + i], im[4 * b + i]);
   }
-  dft16x4_mfma(re, im, ops);
+  dft16x4_mfma(re, im, dft16_ops<false>(optab));
   const __amdgpu_buffer_rsrc_t rh =
       buffer_rsrc(H + (size_t)(tile / M::TILES) * hstride + (size_t)tt * M::HTILE, M::HTILE * sizeof(float2));
   const unsigned hoff = (unsigned)(4 * g * 256 * GM + 64 * w + j) * (unsigned)sizeof(float2);
@@ -486,44 +487,51 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
                                   : buffer_ld2(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2)));
   const int e2 = fp16_exp(block_max<Q::WAVES>(absmax16(re, im), red[1]));  // also: F read everywhere
   scale_soa(re, im, ldexpf(1.0f, e2));
-  ops = dft16_ops<true>(optab);
-  dft16x4_mfma(re, im, ops);  // -> y2 = 4g+i
+  dft16x4_mfma(re, im, dft16_ops<true>(optab));  // -> y2 = 4g+i
+  // The inverse half recomputes the thread's indices from its id, kept opaque
+  // to the compiler, rather than holding the forward half's values across
+  // the pass (which made the kernel spill to scratch at 128 VGPRs).
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int j2 = tid % 16, g2 = (tid % 64) / 16, w2 = tid / 64, zq2 = w2 / GM, line2 = (w2 % GM) * 16 + j2;
+  const unsigned voff2 =
+      (unsigned)K::mid_index(x0 + line2 % GM, k1, line2 / GM, 4 * zq2 + 64 * g2) * (unsigned)sizeof(float2);
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int beta = 16 * (4 * w + b) + j, kz = beta / GM, xb = beta % GM;
+    const int beta = 16 * (4 * w2 + b) + j2, kz = beta / GM, xb = beta % GM;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) lds_put(lds, kz * Q::F_KZ + (4 * g + i) * GM + xb, re[4 * b + i], im[4 * b + i]);
+    for (int i = 0; i < 4; ++i) lds_put(lds, kz * Q::F_KZ + (4 * g2 + i) * GM + xb, re[4 * b + i], im[4 * b + i]);
   }
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      lds_get(lds, (4 * zq + b + 16 * (4 * g + i)) * Q::F_KZ + line, re[4 * b + i], im[4 * b + i]);
+      lds_get(lds, (4 * zq2 + b + 16 * (4 * g2 + i)) * Q::F_KZ + line2, re[4 * b + i], im[4 * b + i]);
   // inverse z: stage A over m = k1' for t = k0 = 4zq+b -> q = 4g+i, twiddle conj W256^(t q)
-  dft16x4_mfma(re, im, ops);
+  dft16x4_mfma(re, im, dft16_ops<true>(optab));
   __syncthreads();  // F read everywhere before E overwrites it
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      cmulw_soa<true>(re[4 * b + i], im[4 * b + i], tw_s[(4 * zq + b) * (4 * g + i)]);
-      lds_put(lds, (4 * zq + b) * Q::E_ZT + (4 * g + i) * Q::E_K0 + line, re[4 * b + i], im[4 * b + i]);
+      cmulw_soa<true>(re[4 * b + i], im[4 * b + i], tw_s[(4 * zq2 + b) * (4 * g2 + i)]);
+      lds_put(lds, (4 * zq2 + b) * Q::E_ZT + (4 * g2 + i) * Q::E_K0 + line2, re[4 * b + i], im[4 * b + i]);
     }
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      lds_get(lds, (4 * g + i) * Q::E_ZT + (4 * zq + b) * Q::E_K0 + line, re[4 * b + i], im[4 * b + i]);
+      lds_get(lds, (4 * g2 + i) * Q::E_ZT + (4 * zq2 + b) * Q::E_K0 + line2, re[4 * b + i], im[4 * b + i]);
   // stage B over t for q = 4zq+b -> r = 4g+i: z = q + 16 r; six stages of 1/16
-  dft16x4_mfma(re, im, ops);
+  dft16x4_mfma(re, im, dft16_ops<true>(optab));
   const float un = ldexpf(1.0f, 24 - e1 - e2);
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      buffer_st2(rout, voff, (b + 16 * i) * ZB, make_float2(re[4 * b + i] * un, im[4 * b + i] * un));
+      buffer_st2(rout, voff2, (b + 16 * i) * ZB, make_float2(re[4 * b + i] * un, im[4 * b + i] * un));
 }
 
 }  // namespace cfk
