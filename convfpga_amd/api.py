@@ -382,8 +382,8 @@ def rel_l2_device(test, ref, npts, ref_is_f64, stream=None):
 
 
 def profile_begin(max_calls, every=1):
-    """Per-pass events on every `every`-th of the next cf_conv3d_device calls,
-    up to max_calls recorded calls."""
+    """Per-pass events on the last of every group of `every` of the next
+    cf_conv3d_device calls, up to max_calls recorded calls."""
     _check(lib().cf_profile_begin(max_calls, every), "cf_profile_begin")
 
 

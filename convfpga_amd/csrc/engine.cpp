@@ -149,7 +149,7 @@ struct Engine {
   // cf_profile_begin/end: events of up to prof_max calls, (CF_MAX_PASSES+1) each
   std::vector<hipEvent_t> prof_ev;
   unsigned prof_max = 0, prof_calls = 0, prof_npass = 0;
-  unsigned prof_every = 1, prof_seen = 0;  // record every prof_every-th call while armed
+  unsigned prof_every = 1, prof_seen = 0;  // record the last call of every group of prof_every while armed
   std::vector<std::string> prof_names;
   std::vector<double> prof_bytes;
 };
@@ -1441,7 +1441,9 @@ static int conv_device_impl(cf_spectrum_t h, const void* d_sig, void* d_out, uns
     }
   } wr{work, s};
   if (!times) {
-    if (g.prof_max > g.prof_calls && g.prof_seen++ % g.prof_every == 0) {  // armed by cf_profile_begin
+    // armed by cf_profile_begin: the last call of every group of prof_every (a
+    // call that follows others in the stream, not one that starts from an idle GPU)
+    if (g.prof_max > g.prof_calls && ++g.prof_seen % g.prof_every == 0) {
       hipEvent_t* ev = &g.prof_ev[(size_t)g.prof_calls * (CF_MAX_PASSES + 1)];
       CFCHK(run_passes(p, n, how_many, s, f->tw, ev));
       if (g.prof_calls == 0) {

@@ -201,10 +201,10 @@ static_assert(cfk::Xy1MfmaGeo<2, 4>::THREADS == 512 && cfk::MidMfmaGeo<4, 2>::TH
 // ---- split-FP16 MFMA form of the 512^3 axis passes (split_mfma512.h) ----------------
 // natural layout (spectra of CF_PLAN_AXIS5) and the z-blocked intermediate
 // (spectra of CF_PLAN_AXIS5Z); 2 work-groups of 512 threads per CU for the columns
-extern "C" __global__ __launch_bounds__(256) void cf_row_fwd_512_s(CF_ARGS) {
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_row_fwd_512_s(CF_ARGS) {
   cfk::row512_mfma_kernel<false>(in, out, tw, scale, ntotal);
 }
-extern "C" __global__ __launch_bounds__(256) void cf_row_inv_512_s(CF_ARGS) {
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cf_row_inv_512_s(CF_ARGS) {
   cfk::row512_mfma_kernel<true>(in, out, tw, scale, ntotal);
 }
 #define CF_COL512_S(NAME, AXIS, MODE, IZ, OZ)                                                                 \

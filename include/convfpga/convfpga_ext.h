@@ -165,8 +165,9 @@ int cf_conv3d_share_timed(cf_spectrum_t h, const void *d_sig, void *d_out, unsig
 
 /* Event profiling of the next cf_conv3d_device calls without a host sync per
  * call: cf_profile_begin arms it for up to max_calls recorded calls, taking
- * every `every`-th call (1: each one; the others run without events, whose
- * packets between the passes cost the GPU a few microseconds each);
+ * the last call of every group of `every` (1: each one; the others run without
+ * events, whose packets between the passes cost the GPU a few microseconds
+ * each);
  * cf_profile_end waits for them and returns each pass's mean device time over
  * the recorded calls (times->ms, from events recorded on the launch stream
  * between the passes) and the number of calls recorded. */
