@@ -70,6 +70,38 @@ def parse():
     return ap.parse_args()
 
 
+def pocketfft_leg(n, sig, H, reps, threads, ref):
+    """The same FP32 convolution (forward FFT, product with the given spectrum,
+    unnormalised inverse) through scipy's pocketfft, an FFTW-class library,
+    on as many threads: how far the oracle's radix-2 FFT is from a tuned CPU
+    FFT on this host; its output is compared with the oracle's (ref).  None
+    when scipy is not importable."""
+    try:
+        import scipy.fft as sf
+    except ImportError:
+        return None
+    import numpy as np
+    x = sig.reshape(n, n, n)
+    h = H.reshape(n, n, n)
+
+    def conv():
+        y = sf.fftn(x, workers=threads)
+        y *= h
+        return sf.ifftn(y, workers=threads, norm="forward")
+    conv()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = conv()
+        times.append(time.perf_counter() - t0)
+    assert out.dtype == np.complex64
+    r = ref.reshape(n, n, n)
+    err = float(np.linalg.norm((out - r).ravel()) / np.linalg.norm(r.ravel()))
+    return {"value": 1.0 / statistics.median(times), "unit": "conv/s", "cores": threads, "rel_l2_vs_oracle_f32": err,
+            "library": "scipy.fft (pocketfft) %s, complex64" % __import__("scipy").__version__,
+            "sample": "%d reps of the same %d^3 convolution, median" % (reps, n)}
+
+
 def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps, split_out_host=None):
     """Times the oracle's FP32 CPU convolution (port of src/fft_conv3D.cpp:50-177)
     on `reps` convolutions of one cube, and checks the GPU output on the same
@@ -103,6 +135,7 @@ def cpu_baseline_leg(n, gpu_out_host, seed_sig, seed_filt, reps, split_out_host=
                   % (reps, n, "present" if O.fftw_available() else "not installed"),
         "value_1_thread": 1.0 / statistics.median(t1), "cpu_model": cpu_model(),
     }
+    res["fftw_class"] = pocketfft_leg(n, sig, H, reps, O.max_threads(), out)
     check = {}
     if gpu_out_host is not None:
         first = np.ascontiguousarray(gpu_out_host[:P])

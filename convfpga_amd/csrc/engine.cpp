@@ -795,6 +795,38 @@ int autotune(unsigned n, SizeFns* f) {
       best_plan = p;
     }
   }
+  // The plans within 3% of the best are timed again, three interleaved rounds
+  // of 10 convolutions each, and ranked by their mean: near-equal plans are
+  // not decided by one short measurement.
+  std::vector<int> close;
+  for (int p : cand)
+    if (f->tune_ms[p] <= best * 1.03f) close.push_back(p);
+  if (rc == CF_OK && close.size() > 1) {
+    float acc[CF_NUM_PLANS] = {};
+    for (int round = 0; round < 3 && rc == CF_OK; ++round) {
+      for (int p : close) {
+        if (rc != CF_OK) break;
+        PassList v;
+        conv_passes(*f, p, n, 1, sig, out, work, H, 0, 1.0f, v);
+        for (int w = 0; w < 2 && rc == CF_OK; ++w) rc = run_passes(v, n, 1, s, f->tw, nullptr);
+        if (rc == CF_OK && hipEventRecord(g.ev[0], s) != hipSuccess) rc = fail(CF_ERR_RUNTIME, "event");
+        for (int r = 0; r < 10 && rc == CF_OK; ++r) rc = run_passes(v, n, 1, s, f->tw, nullptr);
+        if (rc == CF_OK && (hipEventRecord(g.ev[1], s) != hipSuccess || hipEventSynchronize(g.ev[1]) != hipSuccess))
+          rc = fail(CF_ERR_RUNTIME, "event");
+        if (rc == CF_OK) acc[p] += (float)(ms_between(g.ev[0], g.ev[1]) / 10.0);
+      }
+    }
+    if (rc == CF_OK) {
+      best = 0.0f;
+      for (int p : close) {
+        f->tune_ms[p] = acc[p] / 3.0f;
+        if (best == 0.0f || f->tune_ms[p] < best) {
+          best = f->tune_ms[p];
+          best_plan = p;
+        }
+      }
+    }
+  }
   if (rc == CF_OK) rc = release_work(s);
   cleanup();
   if (rc != CF_OK) return rc;
