@@ -26,8 +26,8 @@ c3_64x64 (64 cubes of 64^3, one shared spectrum), c4_mixed (16x32^3 + 8x64^3 +
 4x128^3 + 1x256^3, own spectra, shuffled), each comparing the persistent
 one-launch kernel, the recorded graph and, where it applies, batched passes;
 c5_split (4x256^3 + 1x512^3, own spectra, U[-1,1) and log-uniform inputs): the
-split-FP16 MFMA form beside the FP32 form, rel-L2 of every instance against
-the FP64 oracle.
+split-FP16 MFMA form beside the FP32 form at both edges, rel-L2 of every
+instance against the FP64 oracle.
 """
 import argparse
 import json
