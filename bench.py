@@ -10,10 +10,11 @@ scale.  Inputs are synthetic (seeded U[-1,1) re/im, generated in HBM by the
 library).  Prints one JSON line.
 
 Per-pass device times come from HIP events recorded by the library on the
-launch stream during the timed loop, on one step in PROFILE_EVERY, or on
-two steps of a loop shorter than 2 * PROFILE_EVERY (the event packets between
-the passes of an instrumented step cost the GPU tens of microseconds, so
-instrumenting every step would slow the loop it measures); the roofline
+launch stream during the timed loop, on the last step of every group of
+PROFILE_EVERY, or on the last step of a loop shorter than 2 * PROFILE_EVERY
+(the event packets between the passes of an instrumented step cost the GPU
+tens of microseconds, so instrumenting every step would slow the loop it
+measures); the roofline
 entry is for the pass with the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
 convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
@@ -254,10 +255,10 @@ def run_single(a, torch, C, api):
     for _ in range(a.warmup):
         C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
-    # per-pass HIP events on every PROFILE_EVERY-th step of the timed loop, or
-    # on two steps of a shorter loop (an instrumented step costs the GPU tens
-    # of microseconds: the event packets between its passes)
-    every = PROFILE_EVERY if a.steps >= 2 * PROFILE_EVERY else max(1, a.steps // 2)
+    # per-pass HIP events on the last step of every group of PROFILE_EVERY in
+    # the timed loop, or on the last step of a shorter loop (an instrumented
+    # step costs the GPU tens of microseconds: the event packets between its passes)
+    every = PROFILE_EVERY if a.steps >= 2 * PROFILE_EVERY else a.steps
     recorded = a.steps // every  # the last step of each group of `every`
     api.profile_begin(recorded, every)
     t0 = time.perf_counter()
