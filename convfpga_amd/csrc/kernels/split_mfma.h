@@ -439,6 +439,19 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
       re[4 * b + i] = t.x;
       im[4 * b + i] = t.y;
     }
+  // the spectrum values, loaded with the points (their latency hidden behind the forward half)
+  float2 hreg[16];
+  {
+    const __amdgpu_buffer_rsrc_t rh =
+        buffer_rsrc(H + (size_t)(tile / M::TILES) * hstride + (size_t)tt * M::HTILE, M::HTILE * sizeof(float2));
+    const unsigned hoff = (unsigned)(4 * g * 256 * GM + 64 * w + j) * (unsigned)sizeof(float2);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        hreg[4 * b + i] = ntotal <= M::TILES ? buffer_ld2_nt(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2))
+                                             : buffer_ld2(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2));
+  }
   const int e1 = fp16_exp(block_max<Q::WAVES>(absmax16(re, im), red[0]));
   scale_soa(re, im, ldexpf(1.0f, e1));
   // (the operand table is re-read before each stage rather than held in 16
@@ -475,16 +488,11 @@ __device__ __forceinline__ void mid_mfma_kernel(const float2* in, float2* out, c
     for (int i = 0; i < 4; ++i) lds_get(lds, kz * Q::F_KZ + (4 * g + i) * GM + xb, re[4 * b + i], im[4 * b + i]);
   }
   dft16x4_mfma(re, im, dft16_ops<false>(optab));
-  const __amdgpu_buffer_rsrc_t rh =
-      buffer_rsrc(H + (size_t)(tile / M::TILES) * hstride + (size_t)tt * M::HTILE, M::HTILE * sizeof(float2));
-  const unsigned hoff = (unsigned)(4 * g * 256 * GM + 64 * w + j) * (unsigned)sizeof(float2);
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int i = 0; i < 4; ++i)  // H at k2 = 4g+i, column beta = 16(4w+b) + j
-      cmul_soa(re[4 * b + i], im[4 * b + i],
-               ntotal <= M::TILES ? buffer_ld2_nt(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2))
-                                  : buffer_ld2(rh, hoff, (i * 256 * GM + 16 * b) * sizeof(float2)));
+      cmul_soa(re[4 * b + i], im[4 * b + i], hreg[4 * b + i]);
   const int e2 = fp16_exp(block_max<Q::WAVES>(absmax16(re, im), red[1]));  // also: F read everywhere
   scale_soa(re, im, ldexpf(1.0f, e2));
   dft16x4_mfma(re, im, dft16_ops<true>(optab));  // -> y2 = 4g+i
