@@ -183,8 +183,10 @@ int cf_profile_end(cf_pass_times_t *avg, unsigned *ncalls);
  * loads it, held until fpga_final()), so the call allocates and frees
  * nothing: it only launches the passes on `stream`.  Limit: an intermediate
  * of at most CF_NOALLOC_SCRATCH_BYTES (N^3 * 8 * how_many bytes: one 256^3
- * cube, 8 at 128^3, 64 at 64^3) -- larger calls fail with CF_ERR_SIZE; plans
- * without an intermediate (CF_PLAN_AXIS5, CF_PLAN_PLANE3) have no limit.
+ * cube, 8 at 128^3, 64 at 64^3; the plans with two intermediates, the o and
+ * h four-step forms, need twice that when d_sig == d_out) -- larger calls
+ * fail with CF_ERR_SIZE; plans without an intermediate (CF_PLAN_AXIS5,
+ * CF_PLAN_PLANE3) have no limit.
  * Every call shares the one scratch array: calls that overlap in time on the
  * GPU (on different streams, or from different threads) race.  The spectrum
  * must have been created in the current session (its edge's kernels and

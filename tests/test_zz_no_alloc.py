@@ -62,7 +62,8 @@ def test_noalloc_matches_conv3d_device(gpu, oracle):
 def test_noalloc_limit_and_plans_without_intermediate(gpu, oracle):
     import torch
     api = gpu.api
-    print("case noalloc size limit (9 x 128^3) and the plane plan at 64^3", flush=True)
+    print("case noalloc size limit (9 x 128^3; 5 x 128^3 in place with two intermediates) and the plane plan at 64^3",
+          flush=True)
     n = 128
     api.set_plan(n, api.CF_PLAN_SPLIT3_G24M2)
     try:
@@ -72,6 +73,20 @@ def test_noalloc_limit_and_plans_without_intermediate(gpu, oracle):
     nine = torch.zeros(9 * n ** 3, dtype=torch.complex64, device="cuda")
     with pytest.raises(gpu.ConvError, match="scratch"):
         api.conv3d_device_noalloc(spec, nine, nine, how_many=9)  # 144 MiB of intermediate
+    # two intermediates (the h form) in place: 2 x 16 MiB per cube, 8 cubes fit but not 5 in place
+    api.set_plan(n, api.CF_PLAN_SPLIT3_G24M2H)
+    try:
+        sh = gpu.Spectrum(n, _t(oracle.uniform(n ** 3, 82004)))
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    five = nine[: 5 * n ** 3]
+    with pytest.raises(gpu.ConvError, match="scratch"):
+        api.conv3d_device_noalloc(sh, five, five, how_many=5)  # 160 MiB of intermediates in place
+    four = _t(oracle.uniform(4 * n ** 3, 82005))
+    want4 = torch.empty_like(four)
+    gpu.conv3d_device(sh, four, want4, how_many=4)
+    api.conv3d_device_noalloc(sh, four, four, how_many=4)  # 128 MiB in place: fits
+    assert _same(four, want4)
     m = 64
     api.set_plan(m, api.CF_PLAN_PLANE3)  # no intermediate: any count
     try:
