@@ -10,11 +10,10 @@ scale.  Inputs are synthetic (seeded U[-1,1) re/im, generated in HBM by the
 library).  Prints one JSON line.
 
 Per-pass device times come from HIP events recorded by the library on the
-launch stream during the timed loop, on the last step of every group of
-PROFILE_EVERY, or on the last step of a loop shorter than 2 * PROFILE_EVERY
-(the event packets between the passes of an instrumented step cost the GPU
-tens of microseconds, so instrumenting every step would slow the loop it
-measures); the roofline
+launch stream during the timed loop, on one step in PROFILE_EVERY, or on
+one step of a loop shorter than 2 * PROFILE_EVERY (the event packets between
+the passes of an instrumented step cost the GPU tens of microseconds, so
+instrumenting every step would slow the loop it measures); the roofline
 entry is for the pass with the largest time.  The cpu_baseline leg times the oracle's FP32 CPU
 convolution (OpenMP, all host threads) on a bounded sample of the same
 workload and, on the same inputs, checks the GPU result against it and
@@ -213,8 +212,9 @@ KERNEL_OF_PASS = {  # kernel of each pass name of the axis and plane schedules (
 
 
 def kernel_of_pass(name, n, plan_name):
-    """Kernel of pass `name` under a plan: four-step plans split3_g<GA><GB>[m<GM>][o|h]
-    (o: second intermediate of x group GM; h: y2 pairs interleaved)."""
+    """Kernel of pass `name` under a plan: four-step plans split3_g<GA><GB>[m<GM>][o|h|h<GBO>]
+    (o: second intermediate of x group GM; h: y2 pairs interleaved; h<GBO>: the
+    same in x groups of GBO)."""
     if plan_name.startswith("split3_"):
         v = plan_name.split("_")[1]
         g = v.split("m")[0]
@@ -223,6 +223,8 @@ def kernel_of_pass(name, n, plan_name):
             gi = g[:-1] + v[-2]
         elif v.endswith("h"):
             gi = g + "h"
+        elif "h" in v:  # h<GBO>
+            gi = g[:-1] + v.split("h")[1] + "h"
         return {"xy1_fwd": "cf_xy1_fwd_%d_%s" % (n, g), "xy1_inv": "cf_xy1_inv_%d_%s" % (n, gi),
                 "y2z_fwd_mul_inv": "cf_mid_%d_%s" % (n, v)}.get(name, "")
     k = KERNEL_OF_PASS.get(name, "").format(N=n)
@@ -252,15 +254,19 @@ def run_single(a, torch, C, api):
     # datasheet peak, and the calibration dispatch for the HBM byte counters
     copy_gbs = copy_bandwidth(torch, max(P, 1 << 24))
 
+    # Per-pass HIP events on every PROFILE_EVERY-th call (or one call of a
+    # loop shorter than 2 * PROFILE_EVERY) counted from the first warm-up
+    # call, with the interval longer than the warm-up so that only timed steps
+    # are instrumented (an instrumented step costs the GPU tens of
+    # microseconds: the event packets between its passes).  Armed before the
+    # warm-up, so that the GPU goes from the warm-up straight into the timed loop.
+    every = PROFILE_EVERY if a.steps >= 2 * PROFILE_EVERY else a.steps
+    every = max(every, a.warmup + 1)
+    recorded = (a.warmup + a.steps) // every - a.warmup // every
+    api.profile_begin(recorded, every)
     for _ in range(a.warmup):
         C.conv3d_device(spec, sig, out)
     torch.cuda.synchronize()
-    # per-pass HIP events on the last step of every group of PROFILE_EVERY in
-    # the timed loop, or on the last step of a shorter loop (an instrumented
-    # step costs the GPU tens of microseconds: the event packets between its passes)
-    every = PROFILE_EVERY if a.steps >= 2 * PROFILE_EVERY else a.steps
-    recorded = a.steps // every  # the last step of each group of `every`
-    api.profile_begin(recorded, every)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         C.conv3d_device(spec, sig, out)

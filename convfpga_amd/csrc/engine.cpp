@@ -79,16 +79,16 @@ struct SizeFns {
 };
 
 // four-step plans: CF_PLAN_SPLIT3_G22 .. G28M1 are variants 0..8, then
-// CF_PLAN_SPLIT3_G24M2O and G28M2O variants 9 and 10 (geometry.h)
+// CF_PLAN_SPLIT3_G24M2O .. G28M2H4 variants 9..13 (geometry.h)
 bool is_split(int plan) {
   return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) ||
-         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2H);
+         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2H4);
 }
 int split_variant(int plan) {
   return plan <= CF_PLAN_SPLIT3_G28M1 ? plan - CF_PLAN_SPLIT3_G22 : 9 + (plan - CF_PLAN_SPLIT3_G24M2O);
 }
-static_assert(cfgeo::kSplitVariants == 13 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
-                  CF_PLAN_SPLIT3_G28M2H == CF_PLAN_SPLIT3_G24M2O + 3 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2H + 1,
+static_assert(cfgeo::kSplitVariants == 14 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
+                  CF_PLAN_SPLIT3_G28M2H4 == CF_PLAN_SPLIT3_G24M2O + 4 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2H4 + 1,
               "plan ids of the four-step variants");
 // plans whose passes go through the workspace (one cube per convolution in flight)
 bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
@@ -314,7 +314,10 @@ int size_res(unsigned n, SizeFns** out) {
     const bool ybo2 = cfgeo::split_ybo(vi) == 2;
     snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : "cf_xy1_inv_%u_g%d%d", n, ga, gbo);
     CFCHK(get_fn(&s.xy1_inv[vi], name));
-    snprintf(name, sizeof name, ybo2 ? "cf_mid_%u_%sh" : gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
+    if (ybo2 && gbo != gb)
+      snprintf(name, sizeof name, "cf_mid_%u_%sh%d", n, sfx, gbo);
+    else
+      snprintf(name, sizeof name, ybo2 ? "cf_mid_%u_%sh" : gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
     CFCHK(get_fn(&s.mid[vi], name));
     snprintf(name, sizeof name, "cf_midspec_%u_%s", n, sfx);
     CFCHK(get_fn(&s.mspec[vi], name));
@@ -1635,6 +1638,11 @@ extern "C" int cf_profile_begin(unsigned max_calls, unsigned every) {
   // timing-only events: no system-scope fence (cache writeback) when they
   // complete, which would otherwise stall the next pass for microseconds
   for (auto& e : g.prof_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  // each event recorded once here: the runtime sets an event up on its first
+  // record, which would otherwise stall the host inside the profiled calls
+  // (and let the GPU run dry behind it)
+  for (auto e : g.prof_ev) HIPCHK(hipEventRecord(e, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
   g.prof_max = max_calls;
   g.prof_calls = 0;
   g.prof_every = every;

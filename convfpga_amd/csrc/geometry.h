@@ -38,17 +38,17 @@ constexpr int plane_tiles(int n) { return n; }
 // writes (runs of GA*GB points), GM = x values per middle-pass tile (GM
 // divides GB), GBO = x group of the intermediate the middle pass writes and
 // the last pass reads: GB (in place) or GM (out of place, every tile's output
-// one contiguous block).
-constexpr int kSplitVariants = 13;
-//                                       g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o g24m2h g28m2h
-constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8, 4, 8};
-constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2, 2, 2};
-constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2, 4, 8};
+// one contiguous block), or a smaller group with y2 pairs interleaved (h4).
+constexpr int kSplitVariants = 14;
+//                                       g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o g24m2h g28m2h g28m2h4
+constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8, 4, 8, 8};
+constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2, 2, 2, 2};
+constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2, 4, 8, 4};
 // y2 values interleaved into each run of the second intermediate (YBO = 2:
 // middle tiles of 2 x write runs of 4 points, last-pass tiles of 2 y2 read
 // runs of 2*GBO points)
-constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2};
+constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2};
 constexpr int split_ga(int v) { return kSplitGA[v]; }
 constexpr int split_gb(int v) { return kSplitGB[v]; }
 constexpr int split_gm(int v) { return kSplitGM[v]; }

@@ -126,6 +126,12 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__(N * 2) void cf_xy1_inv_##N##_g2##GB##h(CF_ARGS) {                     \
     cfk::xy1_kernel<N, 2, GB, true, 2>(in, out, tw, scale, ntotal);                                            \
   }
+// middle pass reading x groups of 8 (whole-line runs of the first pass) and
+// writing x groups of 4 with y2 pairs interleaved (read by cf_xy1_inv_N_g24h)
+#define CF_SPLIT_H4(N)                                                                                          \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, 8, 2>::THREADS)) void cf_mid_##N##_g28m2h4(CF_ARGS) { \
+    cfk::mid_kernel<N, 2, 8, 2, true, 4, 2>(in, out, H, tw, hstride, ntotal);                                  \
+  }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
 // (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
@@ -147,6 +153,7 @@ CF_SPLIT_BO(256, 2, 4, 2, 2, g24m2o)
 CF_SPLIT_BO(256, 2, 8, 2, 2, g28m2o)
 CF_SPLIT_H(256, 4, g24m2h)
 CF_SPLIT_H(256, 8, g28m2h)
+CF_SPLIT_H4(256)
 CF_SPLIT_AC(128, 2, 2)
 CF_SPLIT_AC(128, 2, 4)
 CF_SPLIT_AC(128, 4, 2)
@@ -165,6 +172,7 @@ CF_SPLIT_BO(128, 2, 4, 2, 2, g24m2o)
 CF_SPLIT_BO(128, 2, 8, 2, 2, g28m2o)
 CF_SPLIT_H(128, 4, g24m2h)
 CF_SPLIT_H(128, 8, g28m2h)
+CF_SPLIT_H4(128)
 template <int N, int V>
 struct SplitGeoCheck {
   using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;

@@ -348,8 +348,9 @@ struct MidPass {
   using K = SplitGeo<N, GA, GB>;
   using KO = SplitGeo<N, GA, GBO, YBO>;
   using M = MidGeoN<N, GB, GM>;
-  static_assert((YBO == 1 && (GBO == GB || GBO == GM)) || (YBO == 2 && GBO == GB),
-                "the output layout is the input's, the tile's own block, or the input's with y2 pairs interleaved");
+  static_assert((YBO == 1 && (GBO == GB || GBO == GM)) || (YBO == 2 && GB % GBO == 0 && GBO % GM == 0),
+                "the output layout is the input's, the tile's own block, or an x group dividing the input's "
+                "(a multiple of the tile's x) with y2 pairs interleaved");
   static constexpr int L = M::LINES, Y2 = M::Y2, T = M::T, RPT = M::RPT;
   struct Regs {
     float2 v[16];

@@ -73,7 +73,11 @@ extern "C" {
  * pass reads runs of 8 (G24M2H) or 16 (G28M2H) */
 #define CF_PLAN_SPLIT3_G24M2H 15
 #define CF_PLAN_SPLIT3_G28M2H 16
-#define CF_NUM_PLANS 17
+/* the first pass writes whole-line runs (x groups of 8, as G28M2H), the middle
+ * pass writes x groups of 4 with y2 pairs interleaved (runs of 4 points, two
+ * tiles per 64-byte run), the last pass reads runs of 8 (as G24M2H) */
+#define CF_PLAN_SPLIT3_G28M2H4 17
+#define CF_NUM_PLANS 18
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */

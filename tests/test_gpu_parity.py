@@ -290,7 +290,7 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
     [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)] + \
-    [(n, p) for n in (128, 256) for p in (13, 14, 15, 16)]
+    [(n, p) for n in (128, 256) for p in (13, 14, 15, 16, 17)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
@@ -374,7 +374,7 @@ def _spectrum_order(api, n, plan, spec64):
         return spec64
     if name == "axis5z":
         return np.ascontiguousarray(spec64.reshape(n, n, n // 16, 16).transpose(2, 1, 0, 3)).reshape(-1)
-    tail = name.split("_g")[1].rstrip("oh")  # <GA><GB>[m<GM>][o|h]
+    tail = name.split("_g")[1].split("o")[0].split("h")[0]  # <GA><GB>[m<GM>][o|h|h4]
     gm = int(tail.split("m")[1]) if "m" in tail else int(tail[1])
     ref = spec64.reshape(n, n // 16, 16, n // gm, gm)  # (kz, k2, k1, kx/GM, kx%GM)
     return np.ascontiguousarray(ref.transpose(2, 3, 1, 0, 4)).reshape(-1)
