@@ -79,16 +79,16 @@ struct SizeFns {
 };
 
 // four-step plans: CF_PLAN_SPLIT3_G22 .. G28M1 are variants 0..8, then
-// CF_PLAN_SPLIT3_G24M2O .. G28M2H4 variants 9..13 (geometry.h)
+// CF_PLAN_SPLIT3_G24M2O .. G28M2OC4 variants 9..14 (geometry.h)
 bool is_split(int plan) {
   return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) ||
-         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2H4);
+         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2OC4);
 }
 int split_variant(int plan) {
   return plan <= CF_PLAN_SPLIT3_G28M1 ? plan - CF_PLAN_SPLIT3_G22 : 9 + (plan - CF_PLAN_SPLIT3_G24M2O);
 }
-static_assert(cfgeo::kSplitVariants == 14 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
-                  CF_PLAN_SPLIT3_G28M2H4 == CF_PLAN_SPLIT3_G24M2O + 4 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2H4 + 1,
+static_assert(cfgeo::kSplitVariants == 15 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
+                  CF_PLAN_SPLIT3_G28M2OC4 == CF_PLAN_SPLIT3_G24M2O + 5 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2OC4 + 1,
               "plan ids of the four-step variants");
 // plans whose passes go through the workspace (one cube per convolution in flight)
 bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
@@ -302,7 +302,7 @@ int size_res(unsigned n, SizeFns** out) {
   for (int vi = 0; vi < cfgeo::kSplitVariants; ++vi) {
     if (!cfgeo::split_supported((int)n, vi)) continue;
     const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
-    const int gbo = cfgeo::split_gbo(vi);
+    const int gbo = cfgeo::split_gbo(vi), gao = cfgeo::split_gao(vi);
     char sfx[16];  // the in-place middle pass of the same input layout and tile (its filter setup too)
     if (gm == gb)
       snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
@@ -312,7 +312,7 @@ int size_res(unsigned n, SizeFns** out) {
     CFCHK(get_fn(&s.xy1_fwd[vi], name));
     // the last pass reads the middle pass's output layout: x group gbo, y2 pairs interleaved for "h"
     const bool ybo2 = cfgeo::split_ybo(vi) == 2;
-    snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : "cf_xy1_inv_%u_g%d%d", n, ga, gbo);
+    snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : "cf_xy1_inv_%u_g%d%d", n, gao, gbo);
     CFCHK(get_fn(&s.xy1_inv[vi], name));
     if (ybo2 && gbo != gb)
       snprintf(name, sizeof name, "cf_mid_%u_%sh%d", n, sfx, gbo);
@@ -476,6 +476,7 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
     const int vi = split_variant(plan);
     const unsigned at = cfgeo::split_a_threads((int)n, vi), aT = cfgeo::split_a_tiles((int)n, vi);
     const unsigned bt = cfgeo::split_b_threads((int)n, vi), bT = cfgeo::split_b_tiles((int)n, vi);
+    const unsigned ct3 = cfgeo::split_c_threads((int)n, vi), cT3 = cfgeo::split_c_tiles((int)n, vi);
     void* w1 = work;  // first intermediate (the middle pass's input)
     void* w2 = work;  // second one (its output): the same cube unless the plan writes out of place
     if (two_intermediates(plan)) {
@@ -488,7 +489,7 @@ size_t conv_passes(const SizeFns& f, int plan, unsigned n, unsigned nb, const vo
     v.push_back({f.xy1_fwd[vi], aT, at, sig, w1, nullptr, 0, 1.0f, "xy1_fwd", io});
     v.push_back({f.mid[vi], bT, bt, w1, w2, H, hstride, 1.0f, "y2z_fwd_mul_inv", io + hbytes});
     const size_t split = v.size();
-    v.push_back({f.xy1_inv[vi], aT, at, w2, out, nullptr, 0, scale, "xy1_inv", io});
+    v.push_back({f.xy1_inv[vi], cT3, ct3, w2, out, nullptr, 0, scale, "xy1_inv", io});
     return split;
   }
   if (plan == CF_PLAN_AXIS5Z) {  // natural -> z-blocked in the workspace -> natural
@@ -1810,11 +1811,12 @@ static int batch_schedule(const cf_instance_t& x, unsigned i, void* w1, void* w2
   if (is_split(plan) && cfgeo::batch_split_supported(split_variant(plan)) && (n == 128 || n == 256)) {
     const int vi = split_variant(plan), nn = (int)n;
     const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi), gbo = cfgeo::split_gbo(vi);
-    const int sa = ga * gb >= 16 ? 1 : 16 / (ga * gb), sc = ga * gbo >= 16 ? 1 : 16 / (ga * gbo);
+    const int gao = cfgeo::split_gao(vi);
+    const int sa = ga * gb >= 16 ? 1 : 16 / (ga * gb), sc = gao * gbo >= 16 ? 1 : 16 / (gao * gbo);
     v.push_back({CF_BK_XY1_FWD, vi, cfgeo::split_a_tiles(nn, vi), cfgeo::split_a_threads(nn, vi), sa, sig, w1, nullptr,
                  1.0f});
     v.push_back({CF_BK_MID, vi, cfgeo::split_b_tiles(nn, vi), cfgeo::split_b_threads(nn, vi), gb / gm, w1, w2, H, 1.0f});
-    v.push_back({CF_BK_XY1_INV, vi, cfgeo::split_a_tiles(nn, vi), cfgeo::split_a_threads(nn, vi), sc, w2, out, nullptr,
+    v.push_back({CF_BK_XY1_INV, vi, cfgeo::split_c_tiles(nn, vi), cfgeo::split_c_threads(nn, vi), sc, w2, out, nullptr,
                  last_scale});
     return CF_OK;
   }

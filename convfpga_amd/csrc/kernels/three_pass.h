@@ -134,6 +134,18 @@ struct SplitGeo {
   // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
   static constexpr int A_COL_PLANE = N * 17 + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
+  // Tiles of 4 y2 values at N = 256 (1024 threads): the rows <-> columns
+  // exchange moves real parts, then imaginary parts, through one float array,
+  // (y2 plane, x, y1) at plane * H_PLANE + x * HQ + y1, 74 KiB instead of
+  // 138, so that two work-groups share a CU instead of one.  HQ = 18 and the
+  // plane offset 3 (mod 64) leave both access patterns (16 x by 4 y1 per wave
+  // for the rows, 16 x by 4 planes for the columns) without bank conflicts in
+  // either half of a wave.
+  static constexpr bool HALF = N == 256 && GA == 4;
+  static constexpr int HQ = 18;
+  static constexpr int H_PLANE = N * HQ + 3;
+  static constexpr int H_LDS = (GA * H_PLANE + 1) / 2;  // in float2
+  static constexpr int LDS = HALF ? H_LDS : A_LDS;      // the pass's LDS (float2)
   // intermediate index of (x, k1, y2, z): [k1][x/GB][z][y2/YB][x%GB][y2%YB].
   // YB = 1: runs of GB x.  YB = 2: pairs of y2 interleaved into each run of
   // GB x, so that an outer tile of 2 y2 values reads runs of 2*GB points while
@@ -253,13 +265,76 @@ struct Xy1Pass {
       }
     }
   }
+  // The half-width exchange (K::HALF), one tile: the row FFTs run in
+  // registers (lane moves, no LDS) and the exchange is made in two halves.
+  __device__ __forceinline__ void process_half(int tile, Regs& r) const {
+    size_t cube;
+    int z, y2_0;
+    tile_pos(tile, cube, z, y2_0);
+    float* const ldsf = reinterpret_cast<float*>(lds);
+    float* const rowp = ldsf + ry2l * K::H_PLANE + rt * K::HQ + ry1;  // x = rt + T*m
+    float* const colp = ldsf + cy2l * K::H_PLANE + ckx * K::HQ;       // y1 / k1 = j
+    constexpr int RS = T * K::HQ;
+    const int cy2 = y2_0 + cy2l;
+    float2 u[16];
+    if constexpr (!INV) {
+      fft_line<N, false, true>(r.v, rt, RowLds{nullptr}, tw_s);  // a row: lanes of one wave
+#pragma unroll
+      for (int m = 0; m < 16; ++m) rowp[RS * m] = r.v[m].x;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[j].x = colp[j];
+      __syncthreads();  // every real part read before the imaginary parts overwrite them
+#pragma unroll
+      for (int m = 0; m < 16; ++m) rowp[RS * m] = r.v[m].y;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; ++j) u[j].y = colp[j];
+      DFT<16, false>::run(u);  // u[k1]
+#pragma unroll
+      for (int k1 = 1; k1 < 16; ++k1) u[k1] = cmulw<false>(u[k1], tw_s[cy2 * k1]);
+      float2* dst = out + cube + K::mid_index(ckx, 0, cy2, z);
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) dst[k1 * K1_STRIDE] = u[k1];
+    } else {
+#pragma unroll
+      for (int k1 = 1; k1 < 16; ++k1) r.v[k1] = cmulw<true>(r.v[k1], tw_s[cy2 * k1]);
+      DFT<16, true>::run(r.v);  // r.v[y1]
+#pragma unroll
+      for (int j = 0; j < 16; ++j) colp[j] = r.v[j].x;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 16; ++m) u[m].x = rowp[RS * m];
+      __syncthreads();  // every real part read before the imaginary parts overwrite them
+#pragma unroll
+      for (int j = 0; j < 16; ++j) colp[j] = r.v[j].y;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 16; ++m) u[m].y = rowp[RS * m];
+      fft_line<N, true, true>(u, rt, RowLds{nullptr}, tw_s);
+      if (nt) {  // the output is not read again by this convolution either
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(out + cube, (unsigned)(N * N * N * sizeof(float2)));
+        const unsigned voff = (unsigned)(((z * N + Y2 * ry1 + y2_0 + ry2l) * N + rt) * sizeof(float2));
+#pragma unroll
+        for (int m = 0; m < 16; ++m) buffer_st2_nt(rs, voff, (unsigned)(T * m * sizeof(float2)), cscale(u[m], scale));
+      } else {
+        float2* dst = out + cube + (size_t)z * N * N + (Y2 * ry1 + y2_0 + ry2l) * N + rt;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dst[T * m] = cscale(u[m], scale);
+      }
+    }
+  }
   // One tile by one set of threads (the phases and their barriers in order).
   __device__ __forceinline__ void process(int tile, Regs& r) const {
-    phase1(tile, r);
-    __syncthreads();  // rows read (forward) / columns written (inverse) by every wave
-    phase2(tile, r);
-    __syncthreads();
-    phase3(tile, r);
+    if constexpr (K::HALF) {
+      process_half(tile, r);
+    } else {
+      phase1(tile, r);
+      __syncthreads();  // rows read (forward) / columns written (inverse) by every wave
+      phase2(tile, r);
+      __syncthreads();
+      phase3(tile, r);
+    }
   }
 };
 
@@ -285,7 +360,7 @@ __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const 
   using P = Xy1Pass<N, GA, GB, INV, YB>;
   constexpr int S = (GA * GB >= 16) ? 1 : 16 / (GA * GB);
   static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
-  __shared__ float2 lds[K::A_LDS];
+  __shared__ float2 lds[K::LDS];
   __shared__ float2 tw_s[N];
   const int x = threadIdx.x;
   const int row = x / K::T;

@@ -77,7 +77,10 @@ extern "C" {
  * pass writes x groups of 4 with y2 pairs interleaved (runs of 4 points, two
  * tiles per 64-byte run), the last pass reads runs of 8 (as G24M2H) */
 #define CF_PLAN_SPLIT3_G28M2H4 17
-#define CF_NUM_PLANS 18
+/* G28M2O with last-pass tiles of 4 y2 values (1024 threads), which read runs
+ * of 4*2 points of the middle tiles' blocks */
+#define CF_PLAN_SPLIT3_G28M2OC4 18
+#define CF_NUM_PLANS 19
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */
