@@ -88,3 +88,25 @@ def test_c5_split_line_checks_every_instance():
             assert len(errs) == 5, (dist, form)
             assert all(math.isfinite(v) and 0 < v <= 1e-5 for v in errs), (dist, form, errs)
     assert d["value"] > 0 and d["cpu_baseline"]["value"] > 0
+
+
+def test_pass_kernel_names_of_every_four_step_plan():
+    """The bench names each pass's kernel (for the per-kernel HBM bytes in
+    profiles/traffic_256.json): the names of every four-step plan must be
+    kernels of the built code object, and every 256^3 one must have its bytes."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from convfpga_amd import api
+    blob = open(api.CODE_OBJECT, "rb").read()
+    traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic_256.json")))
+    plans = [name for name in api.PLAN_NAMES.values() if name.startswith("split3_")]
+    assert len(plans) == 15
+    for n in (128, 256):
+        for plan in plans:
+            for p in ("xy1_fwd", "y2z_fwd_mul_inv", "xy1_inv"):
+                k = bench.kernel_of_pass(p, n, plan)
+                assert (k + "\0").encode() in blob, (plan, p, k)
+                if n == 256:
+                    assert k in traffic, (plan, p, k)
+    assert bench.kernel_of_pass("xy1_inv", 256, "split3_g28m2oc4") == "cf_xy1_inv_256_g42"
+    assert bench.kernel_of_pass("y2z_fwd_mul_inv", 256, "split3_g28m2oc4") == "cf_mid_256_g28m2o"
