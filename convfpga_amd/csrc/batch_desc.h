@@ -37,9 +37,18 @@ enum CfBatchKind : int {
 
 constexpr int kBatchThreads = 512;  // threads of a work-group of the persistent kernel
 constexpr int kBatchMaxStages = 5;  // passes of the longest schedule
-// counters: [0] queue head, [1] error flag, [2] work-groups finished,
-// [3 + s] tiles of stage s completed; each on its own 256-byte line
-// (counter c at unsigned index c * kBatchCtrStride), so that the polls of
-// waiting work-groups do not contend with the queue's atomics
-constexpr int kBatchCounters = 3 + kBatchMaxStages;
+constexpr int kBatchXcds = 16;      // XCD ids the counters provide for (MI355X has 8)
+// counters: [0] work-groups started, [1] error flag, [2] work-groups
+// finished, [3 + s] tiles of stage s completed, then one line per XCD x:
+// the number of work-groups on x (unsigned 0); for each stage s, a 64-bit
+// word at unsigned 2 + 2s: work-groups of x out of stage s (high half) and
+// their tiles (low half); at kBatchXcdClaim + s, work-groups of x past the
+// wait for stage s, and at kBatchXcdReady + s, 1 once the first of them has
+// invalidated x's L2.  Each on its own 256-byte line (counter c at unsigned
+// index c * kBatchCtrStride), so that polls do not contend with the other
+// atomics.
+constexpr int kBatchCounters = 3 + kBatchMaxStages + kBatchXcds;
 constexpr int kBatchCtrStride = 64;
+constexpr int kBatchXcdClaim = 2 + 2 * kBatchMaxStages;
+constexpr int kBatchXcdReady = kBatchXcdClaim + kBatchMaxStages;
+static_assert(kBatchXcdReady + kBatchMaxStages <= kBatchCtrStride, "an XCD's counters fit its line");

@@ -112,7 +112,7 @@ struct Engine {
   hipModule_t mod = nullptr;
   hipStream_t stream = nullptr;
   hipFunction_t twgen = nullptr, dft16gen = nullptr, errp = nullptr, errf = nullptr, fillu = nullptr, filll = nullptr;
-  hipFunction_t batch_run = nullptr, batch_run_3p = nullptr;
+  hipFunction_t batch_run = nullptr, batch_run_3p = nullptr, batch_run_3p_small = nullptr;
   hipStream_t capture = nullptr;  // private stream for graph capture
   std::map<unsigned, SizeFns> sizes;
   // workspace of the host-buffer (drop-in) calls
@@ -210,7 +210,7 @@ void release_all() {
   g.stream = nullptr;
   if (g.mod) (void)hipModuleUnload(g.mod);
   g.mod = nullptr;
-  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_run_3p = nullptr;
+  g.twgen = g.dft16gen = g.errp = g.errf = g.fillu = g.filll = g.batch_run = g.batch_run_3p = g.batch_run_3p_small = nullptr;
   g.scratch = nullptr;  // part of the unloaded module
   g.scratch_bytes = 0;
   if (g.capture) (void)hipStreamDestroy(g.capture);
@@ -1171,6 +1171,7 @@ extern "C" int cfe_initialize(const char* platform_name, const char* path, int u
       get_fn(&g.errf, "cf_errnorm_final") != CF_OK || get_fn(&g.fillu, "cf_fill_uniform") != CF_OK ||
       get_fn(&g.filll, "cf_fill_loguniform") != CF_OK ||
       get_fn(&g.batch_run, "cf_batch_run") != CF_OK || get_fn(&g.batch_run_3p, "cf_batch_run_3p") != CF_OK ||
+      get_fn(&g.batch_run_3p_small, "cf_batch_run_3p_small") != CF_OK ||
       ensure_events() != CF_OK) {
     fprintf(stderr, "convfpga: '%s' is not a convfpga code object (%s)\n", path, g_err.c_str());
     release_all();
@@ -1918,8 +1919,10 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     }
     std::vector<CfBatchSeg> segs;
     int stage_need[kBatchMaxStages] = {};
+    int stage_first[kBatchMaxStages + 1] = {};  // first work item of each stage (stage-major numbering)
     int item = 0;
-    for (int st = 0; st < nstages; ++st)
+    for (int st = 0; st < nstages; ++st) {
+      stage_first[st] = item;
       for (unsigned i = 0; i < count; ++i) {
         if (st >= (int)sched[i].size()) continue;
         const BatchPass& q = sched[i][st];
@@ -1947,6 +1950,8 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
         stage_need[st] += sg.ntiles;
         segs.push_back(sg);
       }
+    }
+    for (int st = nstages; st <= kBatchMaxStages; ++st) stage_first[st] = item;
     if (stall_test) {
       // test support: the last stage waits for one tile more than the one
       // before it has, so its wait runs into the (short) spin limit
@@ -1958,25 +1963,30 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     }
     const int nsegs = (int)segs.size(), nitems = item;
     const size_t seg_bytes = sizeof(CfBatchSeg) * segs.size();
-    hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need);
+    hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need + sizeof stage_first);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctr, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);
     if (e == hipSuccess) e = hipMemcpy(b->d_segs, segs.data(), seg_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy((char*)b->d_segs + seg_bytes, stage_need, sizeof stage_need, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy((char*)b->d_segs + seg_bytes + sizeof stage_need, stage_first, sizeof stage_first,
+                    hipMemcpyHostToDevice);
     // the counters start at zero once; the kernel leaves them at zero for the next launch
     if (e == hipSuccess) e = hipMemset(b->d_ctr, 0, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);
     if (e != hipSuccess) {
       batch_free(b);
       return fail(CF_ERR_NOMEM, "batch descriptor allocation: %s", hipGetErrorString(e));
     }
-    // the kernel specialised to the three-pass set when every instance is in it
-    bool three_pass = true;
+    // the kernel specialised to the three-pass set when every instance is in
+    // it, with half the LDS (more work-groups per CU) when every edge is 32 or 64
+    bool three_pass = true, small = true;
     for (unsigned i = 0; i < count; ++i) {
       const unsigned n = inst[i].h->n;
       const int p = inst[i].h->plan;
       three_pass = three_pass && ((p == CF_PLAN_PLANE3 && (n == 32 || n == 64)) ||
                                   (is_split(p) && split_variant(p) == cfgeo::kBatchSplit[1] && (n == 128 || n == 256)));
+      small = small && n <= 64;
     }
-    hipFunction_t kfn = three_pass ? g.batch_run_3p : g.batch_run;
+    hipFunction_t kfn = !three_pass ? g.batch_run : small ? g.batch_run_3p_small : g.batch_run_3p;
     int per_cu = 1, cus = 1;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBatchThreads, 0) != hipSuccess ||
         per_cu < 1)

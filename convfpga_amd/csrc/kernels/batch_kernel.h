@@ -6,23 +6,30 @@
 // The reference overlaps the phases of consecutive volumes by software
 // pipelining pairs of instances on its FPGA queues
 // (host/src/convfpga_svm_batch.c:241-662).  Here every pass tile of every
-// instance is work of a single queue, ordered by stage (the pass's index in
-// its instance's schedule): all tiles of stage 0 of every instance, then all
-// tiles of stage 1, ...  A work-group takes the next work item (a few
-// consecutive tiles of one pass) with an atomic counter and runs it with the
-// ordinary pass code.
+// instance is work of one list, ordered by stage (the pass's index in its
+// instance's schedule): all tiles of stage 0 of every instance, then all
+// tiles of stage 1, ...  A work item is a few consecutive tiles of one pass;
+// work-group w of G runs the items w, w + G, w + 2G, ... of each stage in
+// turn (a static share: one atomic counter handing out every item measured
+// slower), with the ordinary pass code.
 //
-// Ordering between the stages, one cross-XCD release per work-group per stage:
-// when a work-group takes its first item of a later stage, its waves wait for
-// their own stores, then one thread releases (device scope: the XCD's L2 is
-// written back once for everything the work-group stored in that stage) and
-// adds its count of completed tiles to the stage's counter; it then waits
-// until the previous stage is complete and acquires (device scope) before any
-// wave loads.  No deadlock: a work-group waiting at a stage boundary has
-// published its own tiles, and every tile of the previous stage was taken
-// before any tile of the later one, by work-groups that are running and never
-// wait inside a stage.  The wait is bounded: after `spin_limit` polls it raises
-// the error flag and goes on, so the grid always drains (the host reports it).
+// Ordering between the stages, one cross-XCD release per XCD per stage: when
+// a work-group moves on to a later stage, its waves wait for their own stores
+// (they are then in the XCD's L2), and one thread counts the work-group and
+// its tiles out of the stage on its XCD's counter.  The last work-group of the
+// XCD out of the stage releases at device scope (one write-back of the XCD's
+// L2 for everything stored there in that stage) and adds the XCD's tiles to
+// the stage's counter.  A work-group then waits until the previous stage is
+// complete; the first of its XCD past the wait acquires at device scope (one
+// invalidation of the XCD's L2), the others invalidate their CU's L1 once it
+// has.  The work-groups of each XCD are counted at the start (hardware
+// register XCC_ID), after which every work-group waits until all have
+// started.  No deadlock: every work-group is resident; a work-group waiting
+// at a stage boundary has counted itself out of the earlier stages, and every
+// tile of the previous stage belongs to a work-group that has run it or will,
+// since it waits only for stages before that one.  Every wait is bounded:
+// after `spin_limit` polls it raises the error flag and goes on, so the grid
+// always drains (the host reports it).
 //
 // A work-group has 512 threads.  A pass whose tile needs T < 512 threads runs
 // 512/T tiles side by side, each group of T threads on its own slice of LDS;
@@ -31,7 +38,7 @@
 // barrier, may leave a group idle).
 //
 // The counters reset themselves: the last work-group to finish zeroes the
-// queue head and the stage counters for the next launch (the error flag is
+// start, stage and XCD counters for the next launch (the error flag is
 // cleared by the host when it reads it), so a captured graph needs no memset.
 #pragma once
 #include "../batch_desc.h"
@@ -77,6 +84,9 @@ constexpr int kBatchLds =
          cmax(cmax(batch_split_lds<256, cfgeo::kBatchSplit[0]>(), batch_split_lds<256, cfgeo::kBatchSplit[1]>()),
               cmax(batch_split_lds<128, cfgeo::kBatchSplit[0]>(), batch_split_lds<128, cfgeo::kBatchSplit[1]>())));
 static_assert(cfgeo::kBatchSplitCount == 2, "the LDS bound above covers two four-step variants");
+// the same bound for batches of the plane plan at edges of at most 64 (the
+// plane passes and their z pass): about half, so that more work-groups share a CU
+constexpr int kBatchLdsSmall = cmax(cmax(batch_plane_lds<32>(), batch_plane_lds<64>()), batch_axis_lds<64>());
 
 struct BatchTile {
   const float2* in;
@@ -285,33 +295,74 @@ __device__ __forceinline__ void batch_item(const CfBatchSeg& sg, int t0, int t1,
   }
 }
 
-template <bool GENERAL>
+// MAXN: the largest edge the kernel runs (64: the LDS of the small-edge bound)
+template <bool GENERAL, int MAXN = 256>
 __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs, const int* __restrict__ stage_need,
                                              int nsegs, int nitems, unsigned* ctr, unsigned long long spin_limit) {
-  __shared__ float2 lds[kBatchLds];
+  __shared__ float2 lds[MAXN <= 64 ? kBatchLdsSmall : kBatchLds];
   __shared__ float2 tw_s[256];
   __shared__ int s_item, s_seg, s_stage;
-  unsigned* const head = ctr;                                // queue head
+  // thread 0's state (kept in LDS rather than in vector registers across the loop)
+  __shared__ int s_seq_stage, s_seq_item, s_xcc, s_left;
+  __shared__ unsigned s_xcd_wgs;
+  unsigned* const started = ctr;                             // work-groups started
   unsigned* const error = ctr + 1 * kBatchCtrStride;         // error flag
   unsigned* const finished = ctr + 2 * kBatchCtrStride;      // work-groups finished
   auto stage_ctr = [&](int s) { return ctr + (3 + s) * kBatchCtrStride; };  // tiles of stage s completed
+  auto xcd_line = [&](int x) { return ctr + (3 + kBatchMaxStages + x) * kBatchCtrStride; };
+  auto xcd_stage = [&](int x, int s) { return reinterpret_cast<unsigned long long*>(xcd_line(x) + 2 + 2 * s); };
   // Control flow note: thread 0 works alone only between two barriers (taking
   // the item; publishing and waiting at a stage change) and after the loop;
   // everything else runs on all threads with uniform branches.
   int tw_n = 0;             // edge whose twiddles are in tw_s
   int cur_stage = -1;       // stage of the tiles run since the last publication
   unsigned done_tiles = 0;  // how many (uniform)
-  // thread 0 fetches each next item while the current one runs (the atomic's
-  // latency hidden behind the tiles)
-  int next = threadIdx.x == 0 ? (int)atomicAdd(head, 1u) : 0;
+  // Work-group w takes the items stage_first[s] + w, + G, + 2G, ... of each
+  // stage s in turn (G work-groups; items are numbered stage-major, so a
+  // stage's items are one range).  Thread 0 keeps the position.
+  const int* const stage_first = stage_need + kBatchMaxStages;
+  // Thread 0 also keeps this work-group's XCD, the number of work-groups on
+  // it and the first stage the work-group has not left yet.  At the start
+  // every work-group counts itself into its XCD's total (completed before it
+  // counts itself as started) and waits until all have started, so that the
+  // totals are final before any work-group leaves a stage.
+  if (threadIdx.x == 0) {
+    s_seq_stage = 0;
+    s_seq_item = stage_first[0] + (int)blockIdx.x;
+    s_left = 0;
+    const int xcc = (int)(__builtin_amdgcn_s_getreg((31 << 11) | 20) & (kBatchXcds - 1));  // hwreg XCC_ID
+    s_xcc = xcc;
+    __hip_atomic_fetch_add(xcd_line(xcc), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_fetch_add(started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long spins = 0;
+    while (__hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x) {
+      __builtin_amdgcn_s_sleep(16);
+      if (++spins > spin_limit) {
+        atomicOr(error, 1u);
+        break;
+      }
+    }
+    s_xcd_wgs = __hip_atomic_load(xcd_line(xcc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   for (;;) {
     // (no barrier at the end of an item: this one, after thread 0's region,
     // also frees the LDS and the twiddles of the previous item)
     if (threadIdx.x == 0) {
-      const int item = next;
+      int item = nitems;
+      int seq_stage = s_seq_stage, seq_item = s_seq_item;
+      while (seq_stage < kBatchMaxStages) {
+        if (seq_item < stage_first[seq_stage + 1]) {
+          item = seq_item;
+          seq_item += (int)gridDim.x;
+          break;
+        }
+        if (++seq_stage < kBatchMaxStages) seq_item = stage_first[seq_stage] + (int)blockIdx.x;
+      }
+      s_seq_stage = seq_stage;
+      s_seq_item = seq_item;
       int seg = -1, stage = kBatchMaxStages;
       if (item < nitems) {
-        next = (int)atomicAdd(head, 1u);
         int lo = 0, hi = nsegs - 1;  // segments are sorted by first_item: binary search
         while (lo < hi) {
           const int mid = (lo + hi + 1) / 2;
@@ -330,14 +381,30 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
     const int seg = __builtin_amdgcn_readfirstlane(s_seg);
     const int stage = __builtin_amdgcn_readfirstlane(s_stage);
     if (stage != cur_stage) {
-      // leaving cur_stage: every wave's stores have reached its L2, then one
-      // device-scope release and one count for the whole work-group
+      // leaving cur_stage: every wave's stores have reached its XCD's L2
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
       if (threadIdx.x == 0) {
-        if (cur_stage >= 0 && done_tiles > 0)
-          __hip_atomic_fetch_add(stage_ctr(cur_stage), done_tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // Count the work-group (and its tiles) out of every stage before
+        // `stage` that it has not left yet, on its XCD's counter.  The last
+        // work-group of the XCD out of a stage has seen every other one's
+        // count, each made after that work-group's stores reached the L2, so
+        // one device-scope release by it writes the XCD's L2 back for the
+        // whole stage before it adds the XCD's tiles to the stage's counter.
+        const int xcc = s_xcc, left = s_left;
+        const unsigned xcd_wgs = s_xcd_wgs;
+        for (int s = left; s < stage && s < kBatchMaxStages; ++s) {
+          if (stage_need[s] == 0) continue;  // no such stage in this batch
+          const unsigned mine = s == cur_stage ? done_tiles : 0u;
+          const unsigned long long old = __hip_atomic_fetch_add(xcd_stage(xcc, s), (1ull << 32) | mine,
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((unsigned)(old >> 32) + 1 == xcd_wgs) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(stage_ctr(s), (unsigned)old + mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        if (stage > left) s_left = stage;
         if (item < nitems && stage > 0) {
           // the previous stage of every instance complete (which implies the earlier ones)
           const unsigned need = (unsigned)stage_need[stage - 1];
@@ -351,10 +418,30 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
               break;
             }
           }
-          // one acquire for the work-group: it invalidates the caches this CU
-          // reads through; the other waves follow through the barrier below
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          __builtin_amdgcn_s_waitcnt(0);
+          // The first work-group of the XCD past the wait invalidates the
+          // XCD's L2 (and its CU's L1) with a device-scope acquire, once for
+          // the stage; the others wait for that, then invalidate their CU's L1
+          // only (the XCD's L2 holds no line older than the stage's end).  The
+          // other waves follow through the barrier below.
+          const int xcc = s_xcc;
+          unsigned* const claim = xcd_line(xcc) + kBatchXcdClaim + (stage - 1);
+          unsigned* const ready = xcd_line(xcc) + kBatchXcdReady + (stage - 1);
+          if (__hip_atomic_fetch_add(claim, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(ready, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            spins = 0;
+            while (__hip_atomic_load(ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+              __builtin_amdgcn_s_sleep(2);
+              if (++spins > spin_limit) {
+                atomicOr(error, 1u);
+                break;
+              }
+            }
+            asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1
+            __builtin_amdgcn_s_waitcnt(0);
+          }
         }
       }
       __syncthreads();
@@ -377,8 +464,12 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
         break;
       case 32: batch_item<32, GENERAL>(sg, t0, t1, lds, tw_s); break;
       case 64: batch_item<64, GENERAL>(sg, t0, t1, lds, tw_s); break;
-      case 128: batch_item<128, GENERAL>(sg, t0, t1, lds, tw_s); break;
-      case 256: batch_item<256, GENERAL>(sg, t0, t1, lds, tw_s); break;
+      case 128:
+        if constexpr (MAXN >= 128) batch_item<128, GENERAL>(sg, t0, t1, lds, tw_s);
+        break;
+      case 256:
+        if constexpr (MAXN >= 256) batch_item<256, GENERAL>(sg, t0, t1, lds, tw_s);
+        break;
       default: break;
     }
     done_tiles += (unsigned)(t1 - t0);
@@ -389,8 +480,16 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
   if (threadIdx.x == 0) {
     const unsigned out = __hip_atomic_fetch_add(finished, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (out == gridDim.x - 1) {
-      __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int s = 0; s < kBatchMaxStages; ++s) __hip_atomic_store(stage_ctr(s), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int x = 0; x < kBatchXcds; ++x) {
+        __hip_atomic_store(xcd_line(x), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int s = 0; s < kBatchMaxStages; ++s) {
+          __hip_atomic_store(xcd_stage(x, s), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(xcd_line(x) + kBatchXcdClaim + s, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(xcd_line(x) + kBatchXcdReady + s, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __hip_atomic_store(started, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(finished, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
