@@ -132,7 +132,8 @@ struct SplitGeo {
   static constexpr int A_ROW_PITCH = N + N / 16;
   // (kx, y1) block of one y2: kx stride 17, planes offset so that the two y2
   // planes of a half-wave fall on complementary LDS bank pairs (GA = 2)
-  static constexpr int A_COL_PLANE = N * 17 + (GA == 2 ? 1 : 4);
+  static constexpr int CQ = 17;  // column pitch: slots of one x
+  static constexpr int A_COL_PLANE = N * CQ + (GA == 2 ? 1 : 4);
   static constexpr int A_LDS = (GA * A_COL_PLANE > 16 * GA * A_ROW_PITCH) ? GA * A_COL_PLANE : 16 * GA * A_ROW_PITCH;
   // Tiles of 4 y2 values at N = 256 (1024 threads): the rows <-> columns
   // exchange moves real parts, then imaginary parts, through one float array,
@@ -221,7 +222,7 @@ struct Xy1Pass {
 #pragma unroll
       for (int k1 = 1; k1 < 16; ++k1) r.v[k1] = cmulw<true>(r.v[k1], tw_s[cy2 * k1]);
       DFT<16, true>::run(r.v);  // r.v[y1]
-      float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;  // this thread's 16 k1 / y1 slots
+      float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * K::CQ;  // this thread's 16 k1 / y1 slots
 #pragma unroll
       for (int j = 0; j < 16; ++j) colp[j] = r.v[j];
     }
@@ -229,10 +230,10 @@ struct Xy1Pass {
   __device__ __forceinline__ void phase2(int, Regs& r) const {
     if constexpr (!INV) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1] = r.v[m];
+      for (int m = 0; m < 16; ++m) lds[ry2l * K::A_COL_PLANE + (rt + T * m) * K::CQ + ry1] = r.v[m];
     } else {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * 17 + ry1];
+      for (int m = 0; m < 16; ++m) r.v[m] = lds[ry2l * K::A_COL_PLANE + (rt + T * m) * K::CQ + ry1];
     }
   }
   __device__ __forceinline__ void phase3(int tile, Regs& r) const {
@@ -241,7 +242,7 @@ struct Xy1Pass {
     tile_pos(tile, cube, z, y2_0);
     if constexpr (!INV) {
       const int cy2 = y2_0 + cy2l;
-      const float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * 17;
+      const float2* colp = lds + cy2l * K::A_COL_PLANE + ckx * K::CQ;
       float2 u[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) u[j] = colp[j];
