@@ -1952,6 +1952,17 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       }
     }
     for (int st = nstages; st <= kBatchMaxStages; ++st) stage_first[st] = item;
+    // per work item: its segment and stage (segment | stage << 24)
+    if (segs.size() >= (1u << 24)) {
+      batch_free(b);
+      return fail(CF_ERR_ARG, "too many segments in one persistent batch");
+    }
+    std::vector<int> item_tab(item);
+    for (size_t k = 0; k < segs.size(); ++k) {
+      const CfBatchSeg& sg = segs[k];
+      const int items = (sg.ntiles + sg.per_item - 1) / sg.per_item;
+      for (int j = 0; j < items; ++j) item_tab[sg.first_item + j] = (int)k | (sg.stage << 24);
+    }
     if (stall_test) {
       // test support: the last stage waits for one tile more than the one
       // before it has, so its wait runs into the (short) spin limit
@@ -1963,12 +1974,16 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     }
     const int nsegs = (int)segs.size(), nitems = item;
     const size_t seg_bytes = sizeof(CfBatchSeg) * segs.size();
-    hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need + sizeof stage_first);
+    const size_t tab_bytes = sizeof(int) * item_tab.size();
+    hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need + sizeof stage_first + tab_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctr, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);
     if (e == hipSuccess) e = hipMemcpy(b->d_segs, segs.data(), seg_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy((char*)b->d_segs + seg_bytes, stage_need, sizeof stage_need, hipMemcpyHostToDevice);
     if (e == hipSuccess)
       e = hipMemcpy((char*)b->d_segs + seg_bytes + sizeof stage_need, stage_first, sizeof stage_first,
+                    hipMemcpyHostToDevice);
+    if (e == hipSuccess && tab_bytes)
+      e = hipMemcpy((char*)b->d_segs + seg_bytes + sizeof stage_need + sizeof stage_first, item_tab.data(), tab_bytes,
                     hipMemcpyHostToDevice);
     // the counters start at zero once; the kernel leaves them at zero for the next launch
     if (e == hipSuccess) e = hipMemset(b->d_ctr, 0, sizeof(unsigned) * kBatchCounters * kBatchCtrStride);

@@ -11,7 +11,8 @@
 // tiles of stage 1, ...  A work item is a few consecutive tiles of one pass;
 // work-group w of G runs the items w, w + G, w + 2G, ... of each stage in
 // turn (a static share: one atomic counter handing out every item measured
-// slower), with the ordinary pass code.
+// slower), with the ordinary pass code.  A table gives every item's segment
+// and stage; thread 0 loads its next item's entry while the current one runs.
 //
 // Ordering between the stages, one cross-XCD release per XCD per stage: when
 // a work-group moves on to a later stage, its waves wait for their own stores
@@ -321,6 +322,27 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
   // stage s in turn (G work-groups; items are numbered stage-major, so a
   // stage's items are one range).  Thread 0 keeps the position.
   const int* const stage_first = stage_need + kBatchMaxStages;
+  const int* const item_tab = stage_first + kBatchMaxStages + 1;  // per item: segment | stage << 24
+  (void)nsegs;
+  // thread 0's next item of its share, after the one running
+  auto take = [&]() {
+    int item = nitems;
+    int seq_stage = s_seq_stage, seq_item = s_seq_item;
+    while (seq_stage < kBatchMaxStages) {
+      if (seq_item < stage_first[seq_stage + 1]) {
+        item = seq_item;
+        seq_item += (int)gridDim.x;
+        break;
+      }
+      if (++seq_stage < kBatchMaxStages) seq_item = stage_first[seq_stage] + (int)blockIdx.x;
+    }
+    s_seq_stage = seq_stage;
+    s_seq_item = seq_item;
+    return item;
+  };
+  // Thread 0 takes each next item and loads its table entry while the current
+  // item runs, so that an item starts with no chain of dependent loads.
+  int next_item = nitems, next_entry = 0;
   // Thread 0 also keeps this work-group's XCD, the number of work-groups on
   // it and the first stage the work-group has not left yet.  At the start
   // every work-group counts itself into its XCD's total (completed before it
@@ -344,36 +366,21 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
       }
     }
     s_xcd_wgs = __hip_atomic_load(xcd_line(xcc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    next_item = take();
+    if (next_item < nitems) next_entry = item_tab[next_item];
   }
+  CfBatchSeg sg{};  // the segment of the current item (uniform), reloaded when the segment changes
+  int sg_index = -1;
   for (;;) {
     // (no barrier at the end of an item: this one, after thread 0's region,
     // also frees the LDS and the twiddles of the previous item)
     if (threadIdx.x == 0) {
-      int item = nitems;
-      int seq_stage = s_seq_stage, seq_item = s_seq_item;
-      while (seq_stage < kBatchMaxStages) {
-        if (seq_item < stage_first[seq_stage + 1]) {
-          item = seq_item;
-          seq_item += (int)gridDim.x;
-          break;
-        }
-        if (++seq_stage < kBatchMaxStages) seq_item = stage_first[seq_stage] + (int)blockIdx.x;
-      }
-      s_seq_stage = seq_stage;
-      s_seq_item = seq_item;
-      int seg = -1, stage = kBatchMaxStages;
-      if (item < nitems) {
-        int lo = 0, hi = nsegs - 1;  // segments are sorted by first_item: binary search
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) / 2;
-          if (segs[mid].first_item <= item) lo = mid; else hi = mid - 1;
-        }
-        seg = lo;
-        stage = segs[seg].stage;
-      }
+      const int item = next_item;
       s_item = item;
-      s_seg = seg;
-      s_stage = stage;
+      s_seg = item < nitems ? (next_entry & 0xFFFFFF) : -1;
+      s_stage = item < nitems ? (next_entry >> 24) : kBatchMaxStages;
+      next_item = take();
+      if (next_item < nitems) next_entry = item_tab[next_item];  // in flight while this item runs
     }
     __syncthreads();
     // every lane reads the same values; readfirstlane lets the compiler see it
@@ -450,7 +457,10 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
       done_tiles = 0;
     }
     if (item >= nitems) break;
-    const CfBatchSeg sg = segs[seg];
+    if (seg != sg_index) {
+      sg = segs[seg];
+      sg_index = seg;
+    }
     if (sg.n != tw_n) {  // the twiddles of the item's edge (uniform); kept while the edge stays
       for (int i = threadIdx.x; i < sg.n && i < 256; i += kBatchThreads) tw_s[i] = ((const float2*)sg.tw)[i];
       __syncthreads();
