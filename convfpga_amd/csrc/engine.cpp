@@ -1972,7 +1972,7 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       }
       stage_need[nstages - 2] += 1;
     }
-    const int nsegs = (int)segs.size(), nitems = item;
+    const int nitems = item;
     const size_t seg_bytes = sizeof(CfBatchSeg) * segs.size();
     const size_t tab_bytes = sizeof(int) * item_tab.size();
     hipError_t e = hipMalloc(&b->d_segs, seg_bytes + sizeof stage_need + sizeof stage_first + tab_bytes);
@@ -2012,9 +2012,9 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     unsigned* ctr = b->d_ctr;
     const void* sp = b->d_segs;
     const void* need = (const char*)b->d_segs + seg_bytes;
-    int ns = nsegs, ni = nitems;
+    int ni = nitems;
     unsigned long long sl = spin_limit;
-    void* ba[] = {(void*)&sp, (void*)&need, &ns, &ni, &ctr, &sl};
+    void* ba[] = {(void*)&sp, (void*)&need, &ni, &ctr, &sl};  // the kernels' parameters, in order
     if (hipModuleLaunchKernel(kfn, grid, 1, 1, kBatchThreads, 1, 1, 0, g.capture, ba, nullptr) != hipSuccess)
       rc = fail(CF_ERR_RUNTIME, "batch kernel capture failed");
     rc = batch_capture_end(b, rc);

@@ -299,7 +299,7 @@ __device__ __forceinline__ void batch_item(const CfBatchSeg& sg, int t0, int t1,
 // MAXN: the largest edge the kernel runs (64: the LDS of the small-edge bound)
 template <bool GENERAL, int MAXN = 256>
 __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs, const int* __restrict__ stage_need,
-                                             int nsegs, int nitems, unsigned* ctr, unsigned long long spin_limit) {
+                                             int nitems, unsigned* ctr, unsigned long long spin_limit) {
   __shared__ float2 lds[MAXN <= 64 ? kBatchLdsSmall : kBatchLds];
   __shared__ float2 tw_s[256];
   __shared__ int s_item, s_seg, s_stage;
@@ -323,7 +323,6 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
   // stage's items are one range).  Thread 0 keeps the position.
   const int* const stage_first = stage_need + kBatchMaxStages;
   const int* const item_tab = stage_first + kBatchMaxStages + 1;  // per item: segment | stage << 24
-  (void)nsegs;
   // thread 0's next item of its share, after the one running
   auto take = [&]() {
     int item = nitems;

@@ -237,24 +237,24 @@ extern "C" __global__ void cf_dft16_table_gen(void* table) {
 // ---- persistent batch of mixed sizes (batch_kernel.h) -----------------------------
 // 512 threads and about 76 KiB of LDS: two work-groups per CU, so at most 128 VGPRs
 extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run(
-    const CfBatchSeg* segs, const int* stage_need, int nsegs, int nitems, unsigned* ctr,
+    const CfBatchSeg* segs, const int* stage_need, int nitems, unsigned* ctr,
     unsigned long long spin_limit) {
-  cfk::batch_kernel<true>(segs, stage_need, nsegs, nitems, ctr, spin_limit);
+  cfk::batch_kernel<true>(segs, stage_need, nitems, ctr, spin_limit);
 }
 // the same for batches on the three-pass set only (plane plan at 32 and 64,
 // four-step g24m2h at 128 and 256), its tiles inlined
 extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void cf_batch_run_3p(
-    const CfBatchSeg* segs, const int* stage_need, int nsegs, int nitems, unsigned* ctr,
+    const CfBatchSeg* segs, const int* stage_need, int nitems, unsigned* ctr,
     unsigned long long spin_limit) {
-  cfk::batch_kernel<false>(segs, stage_need, nsegs, nitems, ctr, spin_limit);
+  cfk::batch_kernel<false>(segs, stage_need, nitems, ctr, spin_limit);
 }
 // the same for the plane plan at edges of 32 and 64 only: half the LDS and
 // at most 80 VGPRs, three work-groups per CU (four, at 64 VGPRs, spill and
 // measured slower)
 extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) void cf_batch_run_3p_small(
-    const CfBatchSeg* segs, const int* stage_need, int nsegs, int nitems, unsigned* ctr,
+    const CfBatchSeg* segs, const int* stage_need, int nitems, unsigned* ctr,
     unsigned long long spin_limit) {
-  cfk::batch_kernel<false, 64>(segs, stage_need, nsegs, nitems, ctr, spin_limit);
+  cfk::batch_kernel<false, 64>(segs, stage_need, nitems, ctr, spin_limit);
 }
 
 // ---- FP64 table generation -------------------------------------------------
