@@ -207,12 +207,14 @@ int cf_conv3d_device_noalloc(cf_spectrum_t h, const void *d_sig, void *d_out, un
  * of count 1 -- instances may share a spectrum) prepared once and replayed
  * with one graph launch per call.
  *   CF_BATCH_PERSISTENT: one persistent kernel runs every pass tile of every
- *     instance from a single work queue, stage by stage (all first passes,
- *     then all second passes, ...), each instance on the plan of its spectrum:
- *     CF_PLAN_AXIS5 (N = 16..256), CF_PLAN_PLANE3 (N = 32, 64),
- *     CF_PLAN_SPLIT3_G24M2 or CF_PLAN_SPLIT3_G24M2H (N = 128, 256); other plans:
- *     CF_ERR_ARG.  The batch holds the intermediates of its four-step
- *     instances.
+ *     instance, stage by stage (all first passes, then all second passes,
+ *     ...), each instance on the plan of its spectrum: CF_PLAN_AXIS5
+ *     (N = 16..256), CF_PLAN_PLANE3 (N = 32, 64), CF_PLAN_SPLIT3_G24M2 or
+ *     CF_PLAN_SPLIT3_G24M2H (N = 128, 256); other plans: CF_ERR_ARG.  The
+ *     batch holds the intermediates of its four-step instances.  Its
+ *     work-groups wait for each other, so they must all be resident at once:
+ *     with other work filling the GPU at the same time, the kernel's bounded
+ *     waits expire and cf_batch_check reports the error.
  *   CF_BATCH_GRAPH: the passes of each instance's own plan, recorded in order
  *     into one graph (any edge and plan).
  * An output may alias its own signal.  In CF_BATCH_PERSISTENT mode the
