@@ -312,12 +312,15 @@ int size_res(unsigned n, SizeFns** out) {
     CFCHK(get_fn(&s.xy1_fwd[vi], name));
     // the last pass reads the middle pass's output layout: x group gbo, y2 pairs interleaved for "h"
     const bool ybo2 = cfgeo::split_ybo(vi) == 2;
-    snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : "cf_xy1_inv_%u_g%d%d", n, gao, gbo);
+    const bool k1o = cfgeo::split_k1o(vi);  // second intermediate with k1 inside the x group
+    snprintf(name, sizeof name, ybo2 ? "cf_xy1_inv_%u_g%d%dh" : k1o ? "cf_xy1_inv_%u_g%d%dk" : "cf_xy1_inv_%u_g%d%d", n,
+             gao, gbo);
     CFCHK(get_fn(&s.xy1_inv[vi], name));
     if (ybo2 && gbo != gb)
       snprintf(name, sizeof name, "cf_mid_%u_%sh%d", n, sfx, gbo);
     else
-      snprintf(name, sizeof name, ybo2 ? "cf_mid_%u_%sh" : gbo == gb ? "cf_mid_%u_%s" : "cf_mid_%u_%so", n, sfx);
+      snprintf(name, sizeof name, ybo2 ? "cf_mid_%u_%sh" : gbo == gb ? "cf_mid_%u_%s" : k1o ? "cf_mid_%u_%sok" : "cf_mid_%u_%so",
+               n, sfx);
     CFCHK(get_fn(&s.mid[vi], name));
     snprintf(name, sizeof name, "cf_midspec_%u_%s", n, sfx);
     CFCHK(get_fn(&s.mspec[vi], name));

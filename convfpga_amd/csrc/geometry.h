@@ -54,6 +54,9 @@ constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2
 constexpr int kSplitGAO[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 4};
 constexpr int split_ga(int v) { return kSplitGA[v]; }
 constexpr int split_gao(int v) { return kSplitGAO[v]; }
+// the second intermediate with k1 inside the x group (kernels ...k): G28M2OC4,
+// whose last pass of 4 y2 values reads it faster that way
+constexpr bool split_k1o(int v) { return v == 14; }
 constexpr int split_gb(int v) { return kSplitGB[v]; }
 constexpr int split_gm(int v) { return kSplitGM[v]; }
 constexpr int split_gbo(int v) { return kSplitGBO[v]; }

@@ -132,6 +132,15 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, 8, 2>::THREADS)) void cf_mid_##N##_g28m2h4(CF_ARGS) { \
     cfk::mid_kernel<N, 2, 8, 2, true, 4, 2>(in, out, H, tw, hstride, ntotal);                                  \
   }
+// the out-of-place middle pass writing its tile blocks with k1 inside the x
+// group, and the last pass of 4 y2 values that reads them (plan G28M2OC4)
+#define CF_SPLIT_OC4(N)                                                                                         \
+  extern "C" __global__ __launch_bounds__((cfk::MidGeoN<N, 8, 2>::THREADS)) void cf_mid_##N##_g28m2ok(CF_ARGS) { \
+    cfk::mid_kernel<N, 2, 8, 2, true, 2, 1, true>(in, out, H, tw, hstride, ntotal);                            \
+  }                                                                                                             \
+  extern "C" __global__ __launch_bounds__(N * 4) void cf_xy1_inv_##N##_g42k(CF_ARGS) {                          \
+    cfk::xy1_kernel<N, 4, 2, true, 1, true>(in, out, tw, scale, ntotal);                                       \
+  }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
 // (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
@@ -154,6 +163,7 @@ CF_SPLIT_BO(256, 2, 8, 2, 2, g28m2o)
 CF_SPLIT_H(256, 4, g24m2h)
 CF_SPLIT_H(256, 8, g28m2h)
 CF_SPLIT_H4(256)
+CF_SPLIT_OC4(256)
 CF_SPLIT_AC(128, 2, 2)
 CF_SPLIT_AC(128, 2, 4)
 CF_SPLIT_AC(128, 4, 2)
@@ -173,6 +183,7 @@ CF_SPLIT_BO(128, 2, 8, 2, 2, g28m2o)
 CF_SPLIT_H(128, 4, g24m2h)
 CF_SPLIT_H(128, 8, g28m2h)
 CF_SPLIT_H4(128)
+CF_SPLIT_OC4(128)
 template <int N, int V>
 struct SplitGeoCheck {
   using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;

@@ -120,7 +120,7 @@ __device__ __forceinline__ void plane_kernel(const float2* in, float2* out, cons
 // y2 in [0, Y2 = N/16); every line (x row or z line) is held by T = N/16
 // threads of 16 points
 // ---------------------------------------------------------------------------
-template <int N_, int GA, int GB, int YB = 1>
+template <int N_, int GA, int GB, int YB = 1, bool K1IN = false>
 struct SplitGeo {
   static constexpr int N = N_;
   static constexpr int Y2 = N / 16;               // y2 values: the middle pass's DFT size
@@ -150,22 +150,26 @@ struct SplitGeo {
   // intermediate index of (x, k1, y2, z): [k1][x/GB][z][y2/YB][x%GB][y2%YB].
   // YB = 1: runs of GB x.  YB = 2: pairs of y2 interleaved into each run of
   // GB x, so that an outer tile of 2 y2 values reads runs of 2*GB points while
-  // a middle tile of 2 x writes runs of 4.
+  // a middle tile of 2 x writes runs of 4.  K1IN: k1 inside the x group,
+  // [x/GB][k1][z][y2/YB][x%GB][y2%YB], the 16 k1 planes of a last-pass tile
+  // N*Y2*GB points apart instead of N^3/16.
   __device__ __forceinline__ static int mid_index(int x, int k1, int y2, int z) {
-    return ((((k1 * NXH + x / GB) * N + z) * (Y2 / YB) + y2 / YB) * GB + (x % GB)) * YB + y2 % YB;
+    const int blk = K1IN ? (x / GB) * 16 + k1 : k1 * NXH + x / GB;
+    return (((blk * N + z) * (Y2 / YB) + y2 / YB) * GB + (x % GB)) * YB + y2 % YB;
   }
+  static constexpr int K1_STEP = K1IN ? N * Y2 * GB : NXH * N * Y2 * GB;  // mid_index step of k1
 };
 template <int GA, int GB>
 using Split256 = SplitGeo<256, GA, GB>;
 
 // INV = false: pass A (natural -> intermediate).  INV = true: pass C (which
 // may read an intermediate with interleaved y2 pairs, YB = 2 = GA).
-template <int N, int GA, int GB, bool INV, int YB = 1>
+template <int N, int GA, int GB, bool INV, int YB = 1, bool K1IN = false>
 struct Xy1Pass {
-  using K = SplitGeo<N, GA, GB, YB>;
+  using K = SplitGeo<N, GA, GB, YB, K1IN>;
   static_assert(YB == 1 || (INV && YB == GA), "interleaved y2 pairs: read by the inverse pass of y2 pairs");
   static constexpr int Y2 = K::Y2, T = K::T;
-  static constexpr int K1_STRIDE = K::NXH * N * Y2 * GB;  // mid_index step of k1
+  static constexpr int K1_STRIDE = K::K1_STEP;  // mid_index step of k1
   struct Regs {
     float2 v[16];
   };
@@ -355,10 +359,10 @@ __device__ __forceinline__ int xcd_group_tile(int b) {
   }
 }
 
-template <int N, int GA, int GB, bool INV, int YB = 1>
+template <int N, int GA, int GB, bool INV, int YB = 1, bool K1IN = false>
 __device__ __forceinline__ void xy1_kernel(const float2* in, float2* out, const float2* tw, float scale, int ntotal) {
-  using K = SplitGeo<N, GA, GB, YB>;
-  using P = Xy1Pass<N, GA, GB, INV, YB>;
+  using K = SplitGeo<N, GA, GB, YB, K1IN>;
+  using P = Xy1Pass<N, GA, GB, INV, YB, K1IN>;
   constexpr int S = (GA * GB >= 16) ? 1 : 16 / (GA * GB);
   static_assert(K::A_TILES % (8 * S) == 0, "tile count must be a multiple of 8*S");
   __shared__ float2 lds[K::LDS];
@@ -419,10 +423,10 @@ using MidGeo = MidGeoN<256, GB, GM>;
 // place): the tile's output is one contiguous block of N*Y2*GM points, so it
 // stores whole lines however narrow the tile (the reads stay short runs,
 // shared with the sibling tiles through L2).
-template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1>
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1, bool K1O = false>
 struct MidPass {
   using K = SplitGeo<N, GA, GB>;
-  using KO = SplitGeo<N, GA, GBO, YBO>;
+  using KO = SplitGeo<N, GA, GBO, YBO, K1O>;
   using M = MidGeoN<N, GB, GM>;
   static_assert((YBO == 1 && (GBO == GB || GBO == GM)) || (YBO == 2 && GB % GBO == 0 && GBO % GM == 0),
                 "the output layout is the input's, the tile's own block, or an x group dividing the input's "
@@ -553,11 +557,11 @@ struct MidPass {
   }
 };
 
-template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1>
+template <int N, int GA, int GB, int GM, bool HPF = false, int GBO = GB, int YBO = 1, bool K1O = false>
 __device__ __forceinline__ void mid_kernel(const float2* in, float2* out, const float2* H, const float2* tw,
                                            long long hstride, int ntotal) {
   using M = MidGeoN<N, GB, GM>;
-  using P = MidPass<N, GA, GB, GM, HPF, GBO, YBO>;
+  using P = MidPass<N, GA, GB, GM, HPF, GBO, YBO, K1O>;
   static_assert(M::TILES % (8 * M::SIBLINGS) == 0, "tile count must be a multiple of 8*siblings");
   __shared__ float2 lds[M::LDS];
   __shared__ float2 tw_s[N];

@@ -78,7 +78,8 @@ extern "C" {
  * tiles per 64-byte run), the last pass reads runs of 8 (as G24M2H) */
 #define CF_PLAN_SPLIT3_G28M2H4 17
 /* G28M2O with last-pass tiles of 4 y2 values (1024 threads), which read runs
- * of 4*2 points of the middle tiles' blocks */
+ * of 4*2 points of the middle tiles' blocks, and those blocks ordered with k1
+ * inside the x group */
 #define CF_PLAN_SPLIT3_G28M2OC4 18
 #define CF_NUM_PLANS 19
 

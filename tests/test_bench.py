@@ -108,5 +108,5 @@ def test_pass_kernel_names_of_every_four_step_plan():
                 assert (k + "\0").encode() in blob, (plan, p, k)
                 if n == 256:
                     assert k in traffic, (plan, p, k)
-    assert bench.kernel_of_pass("xy1_inv", 256, "split3_g28m2oc4") == "cf_xy1_inv_256_g42"
-    assert bench.kernel_of_pass("y2z_fwd_mul_inv", 256, "split3_g28m2oc4") == "cf_mid_256_g28m2o"
+    assert bench.kernel_of_pass("xy1_inv", 256, "split3_g28m2oc4") == "cf_xy1_inv_256_g42k"
+    assert bench.kernel_of_pass("y2z_fwd_mul_inv", 256, "split3_g28m2oc4") == "cf_mid_256_g28m2ok"
