@@ -18,7 +18,10 @@ struct CfBatchSeg {
   int first_item;   // queue index of the segment's first work item
   int ntiles;       // tiles of the pass
   int per_item;     // tiles per work item (a multiple of the tiles a work-group runs at once)
-  int stage;        // stage of the pass
+  int xcd_group;    // S > 1: the segment's items are placed so that the S items of
+                    // each group of sibling tiles (tiles that share lines of an
+                    // intermediate) sit 8 apart in the stage's order, i.e. on
+                    // work-groups of one XCD; 1: items in tile order
 };
 static_assert(sizeof(CfBatchSeg) == 64, "CfBatchSeg must be 64 bytes");
 

@@ -1920,7 +1920,25 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       }
       nstages = std::max(nstages, (int)sched[i].size());
     }
+    // the kernel specialised to the three-pass set when every instance is in
+    // it, with half the LDS (more work-groups per CU) when every edge is 32 or 64
+    bool three_pass = true, small = true;
+    for (unsigned i = 0; i < count; ++i) {
+      const unsigned n = inst[i].h->n;
+      const int p = inst[i].h->plan;
+      three_pass = three_pass && ((p == CF_PLAN_PLANE3 && (n == 32 || n == 64)) ||
+                                  (is_split(p) && split_variant(p) == cfgeo::kBatchSplit[1] && (n == 128 || n == 256)));
+      small = small && n <= 64;
+    }
+    hipFunction_t kfn = !three_pass ? g.batch_run : small ? g.batch_run_3p_small : g.batch_run_3p;
+    int per_cu = 1, cus = 1;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBatchThreads, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device) != hipSuccess || cus < 1) cus = 1;
+    const int full_grid = per_cu * cus;  // every work-group resident
     std::vector<CfBatchSeg> segs;
+    std::vector<int> seg_stage;
     int stage_need[kBatchMaxStages] = {};
     int stage_first[kBatchMaxStages + 1] = {};  // first work item of each stage (stage-major numbering)
     int item = 0;
@@ -1940,10 +1958,21 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
         sg.variant = q.variant;
         sg.first_item = item;
         sg.ntiles = q.ntiles;
-        // a work item: the tiles a work-group runs side by side, at least the
-        // sibling tiles that share lines of an intermediate (one L2)
-        sg.per_item = std::max(kBatchThreads / q.tile_threads, q.siblings);
-        sg.stage = st;
+        // A work item: the tiles a work-group runs side by side.  Sibling tiles
+        // that share lines of an intermediate must meet in one L2: when there
+        // are more of them than that, their items are placed 8 apart (one XCD,
+        // run at the same time) if the grid is a multiple of 8 and the
+        // segment's items fill whole groups of 8 siblings; otherwise one item
+        // holds them all and one work-group runs them one after another.
+        const int side = kBatchThreads / q.tile_threads;
+        sg.per_item = std::max(side, q.siblings);
+        sg.xcd_group = 1;
+        if (q.siblings > side && q.siblings % side == 0 && full_grid % 8 == 0 && q.ntiles % side == 0 &&
+            (q.ntiles / side) % (8 * (q.siblings / side)) == 0) {
+          sg.per_item = side;
+          sg.xcd_group = q.siblings / side;
+        }
+        seg_stage.push_back(st);
         if (sg.n >= 32 && sg.ntiles % (kBatchThreads / q.tile_threads) != 0) {  // tiles run in whole groups
           batch_free(b);
           return fail(CF_ERR_RUNTIME, "pass kind %d of N = %d: %d tiles do not fill work-groups", q.kind, sg.n,
@@ -1964,7 +1993,7 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     for (size_t k = 0; k < segs.size(); ++k) {
       const CfBatchSeg& sg = segs[k];
       const int items = (sg.ntiles + sg.per_item - 1) / sg.per_item;
-      for (int j = 0; j < items; ++j) item_tab[sg.first_item + j] = (int)k | (sg.stage << 24);
+      for (int j = 0; j < items; ++j) item_tab[sg.first_item + j] = (int)k | (seg_stage[k] << 24);
     }
     if (stall_test) {
       // test support: the last stage waits for one tile more than the one
@@ -1994,23 +2023,7 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       batch_free(b);
       return fail(CF_ERR_NOMEM, "batch descriptor allocation: %s", hipGetErrorString(e));
     }
-    // the kernel specialised to the three-pass set when every instance is in
-    // it, with half the LDS (more work-groups per CU) when every edge is 32 or 64
-    bool three_pass = true, small = true;
-    for (unsigned i = 0; i < count; ++i) {
-      const unsigned n = inst[i].h->n;
-      const int p = inst[i].h->plan;
-      three_pass = three_pass && ((p == CF_PLAN_PLANE3 && (n == 32 || n == 64)) ||
-                                  (is_split(p) && split_variant(p) == cfgeo::kBatchSplit[1] && (n == 128 || n == 256)));
-      small = small && n <= 64;
-    }
-    hipFunction_t kfn = !three_pass ? g.batch_run : small ? g.batch_run_3p_small : g.batch_run_3p;
-    int per_cu = 1, cus = 1;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBatchThreads, 0) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device) != hipSuccess || cus < 1) cus = 1;
-    const unsigned grid = (unsigned)std::min(nitems, per_cu * cus);
+    const unsigned grid = (unsigned)std::min(nitems, full_grid);
     HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
     unsigned* ctr = b->d_ctr;
     const void* sp = b->d_segs;

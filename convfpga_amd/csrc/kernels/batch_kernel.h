@@ -8,7 +8,9 @@
 // (host/src/convfpga_svm_batch.c:241-662).  Here every pass tile of every
 // instance is work of one list, ordered by stage (the pass's index in its
 // instance's schedule): all tiles of stage 0 of every instance, then all
-// tiles of stage 1, ...  A work item is a few consecutive tiles of one pass;
+// tiles of stage 1, ...  A work item is a few consecutive tiles of one pass
+// (sibling tiles that share lines of an intermediate, when they are not in
+// one item, are placed 8 items apart: on one XCD, at the same time);
 // work-group w of G runs the items w, w + G, w + 2G, ... of each stage in
 // turn (a static share: one atomic counter handing out every item measured
 // slower), with the ordinary pass code.  A table gives every item's segment
@@ -465,7 +467,18 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
       __syncthreads();
       tw_n = sg.n;
     }
-    const int t0 = (item - sg.first_item) * sg.per_item;
+    // The item's place in its segment; with sibling groups of S items, the
+    // items j, j+8, ..., j+8(S-1) run one group: work-group w of a stage runs
+    // its items w, w+G, ... (G a multiple of 8), all on XCD w % 8, and the
+    // siblings then share one L2 while they run at the same time, as in the
+    // pass kernels' tile order (xcd_group_tile).  A bijection on the
+    // segment's items, whose count the host makes a multiple of 8*S.
+    int j = item - sg.first_item;
+    if (sg.xcd_group > 1) {
+      const int S = sg.xcd_group, x = j % 8, r = j / 8;
+      j = ((r / S) * 8 + x) * S + r % S;
+    }
+    const int t0 = j * sg.per_item;
     const int t1 = min(t0 + sg.per_item, sg.ntiles);
     switch (sg.n) {
       case 16:

@@ -73,3 +73,36 @@ def test_stage_counting_model():
         nx = rng.choice([1, 2, 8])
         xcc_of = [rng.randrange(nx) if rng.random() < 0.3 else b % nx for b in range(G)]
         _run(stage_items, tiles, G, xcc_of, rng)
+
+
+def _place(j, S):
+    """Tile group of the segment's item j with sibling groups of S items
+    (batch_kernel.h, the item's place in its segment)."""
+    if S <= 1:
+        return j
+    x, r = j % 8, j // 8
+    return ((r // S) * 8 + x) * S + r % S
+
+
+def test_sibling_placement():
+    """Items of a segment with sibling groups of S: every tile group is run
+    exactly once, and the S items of one sibling group land on work-groups of
+    one XCD (w % 8, with G a multiple of 8) in the same round of the static
+    shares -- whatever the segment's offset in its stage."""
+    rng = random.Random(11)
+    for _ in range(300):
+        S = rng.choice([2, 4])
+        items = 8 * S * rng.randint(1, 40)
+        offset = rng.randint(0, 500)  # first item of the segment within its stage
+        G = 8 * rng.randint(1, 100)
+        placed = [_place(j, S) for j in range(items)]
+        assert sorted(placed) == list(range(items))
+        where = {}
+        for j, t in enumerate(placed):
+            r = offset + j
+            where[t] = ((r % G) % 8, r // G)  # (XCD of the work-group, round)
+        for grp in range(items // S):
+            xcds = {where[grp * S + k][0] for k in range(S)}
+            assert len(xcds) == 1, (S, items, offset, G, grp)
+            rounds = {where[grp * S + k][1] for k in range(S)}
+            assert max(rounds) - min(rounds) <= 1 + (S * 8) // G
