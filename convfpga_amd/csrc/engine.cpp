@@ -1942,9 +1942,15 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     int stage_need[kBatchMaxStages] = {};
     int stage_first[kBatchMaxStages + 1] = {};  // first work item of each stage (stage-major numbering)
     int item = 0;
+    // within a stage, the segments by edge, largest first: a work-group's
+    // items (G apart) then change edge, and restage its twiddles, rarely
+    std::vector<unsigned> by_edge(count);
+    for (unsigned i = 0; i < count; ++i) by_edge[i] = i;
+    std::stable_sort(by_edge.begin(), by_edge.end(),
+                     [&](unsigned a, unsigned c) { return inst[a].h->n > inst[c].h->n; });
     for (int st = 0; st < nstages; ++st) {
       stage_first[st] = item;
-      for (unsigned i = 0; i < count; ++i) {
+      for (const unsigned i : by_edge) {
         if (st >= (int)sched[i].size()) continue;
         const BatchPass& q = sched[i][st];
         CfBatchSeg sg{};

@@ -14,7 +14,8 @@
 // work-group w of G runs the items w, w + G, w + 2G, ... of each stage in
 // turn (a static share: one atomic counter handing out every item measured
 // slower), with the ordinary pass code.  A table gives every item's segment
-// and stage; thread 0 loads its next item's entry while the current one runs.
+// and stage; each work-group lists its share's items and their entries in
+// LDS at the start, so that taking an item waits on no memory load.
 //
 // Ordering between the stages, one cross-XCD release per XCD per stage: when
 // a work-group moves on to a later stage, its waves wait for their own stores
@@ -304,9 +305,8 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
                                              int nitems, unsigned* ctr, unsigned long long spin_limit) {
   __shared__ float2 lds[MAXN <= 64 ? kBatchLdsSmall : kBatchLds];
   __shared__ float2 tw_s[256];
-  __shared__ int s_item, s_seg, s_stage;
-  // thread 0's state (kept in LDS rather than in vector registers across the loop)
-  __shared__ int s_seq_stage, s_seq_item, s_xcc, s_left;
+  // thread 0's state at stage changes (kept in LDS rather than in vector registers across the loop)
+  __shared__ int s_xcc, s_left;
   __shared__ unsigned s_xcd_wgs;
   unsigned* const started = ctr;                             // work-groups started
   unsigned* const error = ctr + 1 * kBatchCtrStride;         // error flag
@@ -314,44 +314,46 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
   auto stage_ctr = [&](int s) { return ctr + (3 + s) * kBatchCtrStride; };  // tiles of stage s completed
   auto xcd_line = [&](int x) { return ctr + (3 + kBatchMaxStages + x) * kBatchCtrStride; };
   auto xcd_stage = [&](int x, int s) { return reinterpret_cast<unsigned long long*>(xcd_line(x) + 2 + 2 * s); };
-  // Control flow note: thread 0 works alone only between two barriers (taking
-  // the item; publishing and waiting at a stage change) and after the loop;
+  // Control flow note: thread 0 works alone only between two barriers
+  // (publishing and waiting at a stage change, the start) and after the loop;
   // everything else runs on all threads with uniform branches.
   int tw_n = 0;             // edge whose twiddles are in tw_s
   int cur_stage = -1;       // stage of the tiles run since the last publication
   unsigned done_tiles = 0;  // how many (uniform)
   // Work-group w takes the items stage_first[s] + w, + G, + 2G, ... of each
   // stage s in turn (G work-groups; items are numbered stage-major, so a
-  // stage's items are one range).  Thread 0 keeps the position.
+  // stage's items are one range).  The shares are static, so the work-group
+  // lists its sequence (item, table entry) in LDS up front, kSeq positions at
+  // a time, and an item starts with one LDS read: any vector load there would
+  // wait on the one in-order vector memory counter, i.e. for the previous
+  // item's stores to complete, before the next item's loads could issue, and
+  // a scalar load would hold up the item's first LDS wait.
+  constexpr int kSeq = kBatchThreads;
+  __shared__ int2 s_seq[kSeq];
   const int* const stage_first = stage_need + kBatchMaxStages;
   const int* const item_tab = stage_first + kBatchMaxStages + 1;  // per item: segment | stage << 24
-  // thread 0's next item of its share, after the one running
-  auto take = [&]() {
-    int item = nitems;
-    int seq_stage = s_seq_stage, seq_item = s_seq_item;
-    while (seq_stage < kBatchMaxStages) {
-      if (seq_item < stage_first[seq_stage + 1]) {
-        item = seq_item;
-        seq_item += (int)gridDim.x;
+  const int G = (int)gridDim.x;
+  // positions base .. base + kSeq - 1 of the share (all threads; past its end: nitems)
+  auto list_share = [&](int base) {
+    int p = base + (int)threadIdx.x, item = nitems;
+    for (int st = 0; st < kBatchMaxStages; ++st) {
+      const int f = stage_first[st] + (int)blockIdx.x, e = stage_first[st + 1];
+      const int cnt = f < e ? (e - f + G - 1) / G : 0;
+      if (p < cnt) {
+        item = f + p * G;
         break;
       }
-      if (++seq_stage < kBatchMaxStages) seq_item = stage_first[seq_stage] + (int)blockIdx.x;
+      p -= cnt;
     }
-    s_seq_stage = seq_stage;
-    s_seq_item = seq_item;
-    return item;
+    s_seq[threadIdx.x] = make_int2(item, item < nitems ? item_tab[item] : 0);
   };
-  // Thread 0 takes each next item and loads its table entry while the current
-  // item runs, so that an item starts with no chain of dependent loads.
-  int next_item = nitems, next_entry = 0;
+  list_share(0);
   // Thread 0 also keeps this work-group's XCD, the number of work-groups on
   // it and the first stage the work-group has not left yet.  At the start
   // every work-group counts itself into its XCD's total (completed before it
   // counts itself as started) and waits until all have started, so that the
   // totals are final before any work-group leaves a stage.
   if (threadIdx.x == 0) {
-    s_seq_stage = 0;
-    s_seq_item = stage_first[0] + (int)blockIdx.x;
     s_left = 0;
     const int xcc = (int)(__builtin_amdgcn_s_getreg((31 << 11) | 20) & (kBatchXcds - 1));  // hwreg XCC_ID
     s_xcc = xcc;
@@ -367,27 +369,25 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
       }
     }
     s_xcd_wgs = __hip_atomic_load(xcd_line(xcc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    next_item = take();
-    if (next_item < nitems) next_entry = item_tab[next_item];
   }
+  using CSeg = const __attribute__((address_space(4))) CfBatchSeg;
+  CSeg* const csegs = (CSeg*)segs;  // read-only here: scalar loads
   CfBatchSeg sg{};  // the segment of the current item (uniform), reloaded when the segment changes
   int sg_index = -1;
-  for (;;) {
-    // (no barrier at the end of an item: this one, after thread 0's region,
-    // also frees the LDS and the twiddles of the previous item)
-    if (threadIdx.x == 0) {
-      const int item = next_item;
-      s_item = item;
-      s_seg = item < nitems ? (next_entry & 0xFFFFFF) : -1;
-      s_stage = item < nitems ? (next_entry >> 24) : kBatchMaxStages;
-      next_item = take();
-      if (next_item < nitems) next_entry = item_tab[next_item];  // in flight while this item runs
+  for (int pos = 0;; ++pos) {
+    if (pos > 0 && pos % kSeq == 0) {  // the next kSeq positions (uniform; every wave has read the last ones)
+      __syncthreads();
+      list_share(pos);
     }
+    // (no barrier at the end of an item: this one frees the LDS and the
+    // twiddles of the previous item and publishes the listed positions; at
+    // the first item it also ends the start)
     __syncthreads();
-    // every lane reads the same values; readfirstlane lets the compiler see it
-    const int item = __builtin_amdgcn_readfirstlane(s_item);
-    const int seg = __builtin_amdgcn_readfirstlane(s_seg);
-    const int stage = __builtin_amdgcn_readfirstlane(s_stage);
+    const int2 e = s_seq[pos % kSeq];
+    const int item = __builtin_amdgcn_readfirstlane(e.x);
+    const int entry = __builtin_amdgcn_readfirstlane(e.y);
+    const int seg = item < nitems ? (entry & 0xFFFFFF) : -1;
+    const int stage = item < nitems ? (entry >> 24) : kBatchMaxStages;
     if (stage != cur_stage) {
       // leaving cur_stage: every wave's stores have reached its XCD's L2
       __builtin_amdgcn_s_waitcnt(0);
@@ -459,7 +459,7 @@ __device__ __forceinline__ void batch_kernel(const CfBatchSeg* __restrict__ segs
     }
     if (item >= nitems) break;
     if (seg != sg_index) {
-      sg = segs[seg];
+      sg = csegs[seg];
       sg_index = seg;
     }
     if (sg.n != tw_n) {  // the twiddles of the item's edge (uniform); kept while the edge stays
