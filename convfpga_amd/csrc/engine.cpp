@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "batch_desc.h"
+#include "batch_spans.h"
 #include "convfpga/convfpga_ext.h"
 #include "engine.h"
 #include "geometry.h"
@@ -1866,20 +1867,21 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       return Span{(const char*)p, (const char*)p + cube_bytes(inst[i].h->n)};
     };
     auto meet = [](Span x, Span y) { return x.a < y.b && y.a < x.b; };
+    std::vector<cfspan::Span> all;
+    all.reserve(2 * (size_t)count);
     for (unsigned i = 0; i < count; ++i) {
       const Span si = span(inst[i].d_sig, i), oi = span(inst[i].d_out, i);
       if (meet(si, oi) && si.a != oi.a)
         return fail(CF_ERR_ARG, "instance %u: output partly overlaps its own signal", i);
-      for (unsigned j = 0; j < count; ++j) {
-        if (j == i) continue;
-        const Span sj = span(inst[j].d_sig, j), oj = span(inst[j].d_out, j);
-        if (meet(oi, sj) || meet(oi, oj))
-          return fail(CF_ERR_ARG,
-                      "instances %u and %u: the output of %u overlaps the buffers of %u; instances of a "
-                      "persistent batch run concurrently (use CF_BATCH_GRAPH for chained convolutions)",
-                      i, j, i, j);
-      }
+      all.push_back({(uintptr_t)oi.a, (uintptr_t)oi.b, i, true});
+      if (si.a != oi.a) all.push_back({(uintptr_t)si.a, (uintptr_t)si.b, i, false});
     }
+    unsigned i = 0, j = 0;
+    if (cfspan::find_conflict(std::move(all), &i, &j))
+      return fail(CF_ERR_ARG,
+                  "instances %u and %u: the output of %u overlaps the buffers of %u; instances of a "
+                  "persistent batch run concurrently (use CF_BATCH_GRAPH for chained convolutions)",
+                  i, j, i, j);
   }
   if (!g.capture) HIPCHK(hipStreamCreateWithFlags(&g.capture, hipStreamNonBlocking));
   auto* b = new cf_batch_s;

@@ -106,3 +106,52 @@ def test_sibling_placement():
             assert len(xcds) == 1, (S, items, offset, G, grp)
             rounds = {where[grp * S + k][1] for k in range(S)}
             assert max(rounds) - min(rounds) <= 1 + (S * 8) // G
+
+
+def _list_share(base, K, first, b, G, nitems):
+    """Positions base .. base+K-1 of work-group b's share, as the kernel lists
+    them in LDS (batch_kernel.h, list_share): per stage, ceil((end - start)/G)
+    items from start = first[s] + b, step G; past the end, nitems."""
+    out = []
+    for t in range(K):
+        p, item = base + t, nitems
+        for s in range(MAX_STAGES):
+            f, e = first[s] + b, first[s + 1]
+            cnt = (e - f + G - 1) // G if f < e else 0
+            if p < cnt:
+                item = f + p * G
+                break
+            p -= cnt
+        out.append(item)
+    return out
+
+
+def test_listed_shares_in_chunks():
+    """The work-group's items listed K positions at a time, chunk after chunk,
+    are exactly its static share in stage order, then nitems."""
+    rng = random.Random(13)
+    for _ in range(300):
+        nst = rng.randint(1, MAX_STAGES)
+        stage_items = [rng.randint(0, 3000) for _ in range(nst)]
+        first = [0]
+        for c in stage_items:
+            first.append(first[-1] + c)
+        first += [first[-1]] * (MAX_STAGES + 1 - len(first))
+        nitems = first[-1]
+        if nitems == 0:
+            continue
+        G = rng.randint(1, min(nitems, 600))
+        K = rng.choice([8, 64, 512])
+        b = rng.randrange(G)
+        share = [it for s in range(MAX_STAGES) for it in range(first[s] + b, first[s + 1], G)]
+        listed = []
+        base = 0
+        while True:
+            chunk = _list_share(base, K, first, b, G, nitems)
+            listed += chunk
+            if chunk[-1] == nitems:
+                break
+            base += K
+        n = len(share)
+        assert listed[:n] == share
+        assert all(x == nitems for x in listed[n:])

@@ -522,6 +522,47 @@ def test_persistent_batch_on_each_plan(gpu, oracle, n, plan, count):
     bg.close()
 
 
+def test_persistent_batch_longer_than_one_listing(gpu, oracle):
+    """A persistent batch with more than 512 work items per work-group (the
+    kernel lists a work-group's items in LDS 512 positions at a time, so this
+    takes a second listing): 16^3 convolutions of one spectrum on the axis
+    plan, five items each, about 600 per work-group at two work-groups per CU;
+    every output bitwise equal to graph mode, a few against the FP64 oracle."""
+    import torch
+    api = gpu.api
+    n = 16
+    P = n ** 3
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    count = min((600 * 2 * cus) // 5, 65536)
+    print("case persistent batch of %d 16^3 instances" % count, flush=True)
+    f = oracle.uniform(P, 87001)
+    api.set_plan(n, api.CF_PLAN_AXIS5)
+    try:
+        spec = gpu.Spectrum(n, _t(f))
+    finally:
+        api.set_plan(n, api.CF_PLAN_AUTO)
+    sig = torch.empty(count * P, dtype=torch.complex64, device="cuda")
+    api.fill_uniform(sig, 2 * count * P, 87002)
+    op = torch.zeros_like(sig)
+    og = torch.zeros_like(sig)
+    nb = 8 * P
+    bp = api.Batch([(sig.data_ptr() + k * nb, op.data_ptr() + k * nb, spec) for k in range(count)],
+                   mode=api.CF_BATCH_PERSISTENT)
+    bg = api.Batch([(sig.data_ptr() + k * nb, og.data_ptr() + k * nb, spec) for k in range(count)],
+                   mode=api.CF_BATCH_GRAPH)
+    bp.launch()
+    bp.check()
+    bg.launch()
+    bg.check()
+    torch.cuda.synchronize()
+    assert torch.equal(op.view(torch.int64), og.view(torch.int64))
+    for k in (0, count // 2, count - 1):
+        ref = oracle.conv3d_f64(n, _np(sig[k * P:(k + 1) * P]), f, 1.0 / P)
+        assert oracle.rel_l2(_np(op[k * P:(k + 1) * P]), ref) <= REL_L2_FP32, k
+    bp.close()
+    bg.close()
+
+
 def test_spectrum_views_outlive_a_closed_parent(gpu, oracle):
     """Closing a two-filter spectrum while views of it are open keeps its
     memory until the last view is closed: a view still convolves correctly."""
