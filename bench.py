@@ -215,7 +215,8 @@ def kernel_of_pass(name, n, plan_name):
     """Kernel of pass `name` under a plan: four-step plans split3_g<GA><GB>[m<GM>][o|h|h<GBO>][c<GAO>]
     (o: second intermediate of x group GM; h: y2 pairs interleaved; h<GBO>: the
     same in x groups of GBO; c<GAO>: last-pass tiles of GAO y2 values, reading a
-    second intermediate with k1 inside the x group)."""
+    second intermediate with k1 inside the x group).  A first pass of GA = 1
+    shares the middle pass of GA = 2 (the middle pass depends on GB only)."""
     if plan_name.startswith("split3_"):
         v = plan_name.split("_")[1]
         v, _, gao = v.partition("c")
@@ -227,10 +228,10 @@ def kernel_of_pass(name, n, plan_name):
             gi = g + "h"
         elif "h" in v:  # h<GBO>
             gi = g[:-1] + v.split("h")[1] + "h"
-        vm = v
+        vm = "g2" + v[2:] if v[1] == "1" else v
         if gao:  # its second intermediate has k1 inside the x group (kernels ...k)
             gi = "g" + gao + gi[2:] + "k"
-            vm = v + "k"
+            vm += "k"
         return {"xy1_fwd": "cf_xy1_fwd_%d_%s" % (n, g), "xy1_inv": "cf_xy1_inv_%d_%s" % (n, gi),
                 "y2z_fwd_mul_inv": "cf_mid_%d_%s" % (n, vm)}.get(name, "")
     k = KERNEL_OF_PASS.get(name, "").format(N=n)

@@ -33,12 +33,14 @@ CF_PLAN_AXIS5Z = 12
 CF_PLAN_SPLIT3_G24M2O, CF_PLAN_SPLIT3_G28M2O, CF_PLAN_SPLIT3_G24M2H, CF_PLAN_SPLIT3_G28M2H = 13, 14, 15, 16
 CF_PLAN_SPLIT3_G28M2H4 = 17
 CF_PLAN_SPLIT3_G28M2OC4 = 18
-CF_NUM_PLANS = 19
+CF_PLAN_SPLIT3_G18M2OC4 = 19
+CF_NUM_PLANS = 20
 CF_BATCH_PERSISTENT, CF_BATCH_GRAPH = 0, 1
 PLAN_NAMES = {1: "axis5", 2: "plane3", 3: "split3_g22", 4: "split3_g24", 5: "split3_g42", 6: "split3_g44",
               7: "split3_g24m2", 8: "split3_g24m1", 9: "split3_g28m4", 10: "split3_g28m2", 11: "split3_g28m1",
               12: "axis5z", 13: "split3_g24m2o", 14: "split3_g28m2o",
-              15: "split3_g24m2h", 16: "split3_g28m2h", 17: "split3_g28m2h4", 18: "split3_g28m2oc4"}
+              15: "split3_g24m2h", 16: "split3_g28m2h", 17: "split3_g28m2h4", 18: "split3_g28m2oc4",
+              19: "split3_g18m2oc4"}
 SUPPORTED_SIZES = (8, 16, 32, 64, 128, 256, 512)
 
 # names of every symbol include/convfpga/*.h declares

@@ -80,16 +80,17 @@ struct SizeFns {
 };
 
 // four-step plans: CF_PLAN_SPLIT3_G22 .. G28M1 are variants 0..8, then
-// CF_PLAN_SPLIT3_G24M2O .. G28M2OC4 variants 9..14 (geometry.h)
+// CF_PLAN_SPLIT3_G24M2O .. G18M2OC4 variants 9..15 (geometry.h)
 bool is_split(int plan) {
   return (plan >= CF_PLAN_SPLIT3_G22 && plan <= CF_PLAN_SPLIT3_G28M1) ||
-         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G28M2OC4);
+         (plan >= CF_PLAN_SPLIT3_G24M2O && plan <= CF_PLAN_SPLIT3_G18M2OC4);
 }
 int split_variant(int plan) {
   return plan <= CF_PLAN_SPLIT3_G28M1 ? plan - CF_PLAN_SPLIT3_G22 : 9 + (plan - CF_PLAN_SPLIT3_G24M2O);
 }
-static_assert(cfgeo::kSplitVariants == 15 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
-                  CF_PLAN_SPLIT3_G28M2OC4 == CF_PLAN_SPLIT3_G24M2O + 5 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G28M2OC4 + 1,
+static_assert(cfgeo::kSplitVariants == 16 && CF_PLAN_SPLIT3_G28M1 - CF_PLAN_SPLIT3_G22 == 8 &&
+                  CF_PLAN_SPLIT3_G28M2OC4 == CF_PLAN_SPLIT3_G24M2O + 5 &&
+                  CF_PLAN_SPLIT3_G18M2OC4 == CF_PLAN_SPLIT3_G28M2OC4 + 1 && CF_NUM_PLANS == CF_PLAN_SPLIT3_G18M2OC4 + 1,
               "plan ids of the four-step variants");
 // plans whose passes go through the workspace (one cube per convolution in flight)
 bool needs_work(int plan) { return is_split(plan) || plan == CF_PLAN_AXIS5Z; }
@@ -304,11 +305,15 @@ int size_res(unsigned n, SizeFns** out) {
     if (!cfgeo::split_supported((int)n, vi)) continue;
     const int ga = cfgeo::split_ga(vi), gb = cfgeo::split_gb(vi), gm = cfgeo::split_gm(vi);
     const int gbo = cfgeo::split_gbo(vi), gao = cfgeo::split_gao(vi);
+    // the middle pass (and its filter setup) depends on the first pass's
+    // layout (x group gb), not on its y2 group: the first pass of one y2
+    // value (ga = 1) shares the kernels of ga = 2
+    const int gam = ga < 2 ? 2 : ga;
     char sfx[16];  // the in-place middle pass of the same input layout and tile (its filter setup too)
     if (gm == gb)
-      snprintf(sfx, sizeof sfx, "g%d%d", ga, gb);
+      snprintf(sfx, sizeof sfx, "g%d%d", gam, gb);
     else
-      snprintf(sfx, sizeof sfx, "g%d%dm%d", ga, gb, gm);
+      snprintf(sfx, sizeof sfx, "g%d%dm%d", gam, gb, gm);
     snprintf(name, sizeof name, "cf_xy1_fwd_%u_g%d%d", n, ga, gb);
     CFCHK(get_fn(&s.xy1_fwd[vi], name));
     // the last pass reads the middle pass's output layout: x group gbo, y2 pairs interleaved for "h"

@@ -39,24 +39,24 @@ constexpr int plane_tiles(int n) { return n; }
 // divides GB), GBO = x group of the intermediate the middle pass writes and
 // the last pass reads: GB (in place) or GM (out of place, every tile's output
 // one contiguous block), or a smaller group with y2 pairs interleaved (h4).
-constexpr int kSplitVariants = 15;
-//                                       g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o g24m2h g28m2h g28m2h4 g28m2oc4
-constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8, 4, 8, 8, 8};
-constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2, 2, 2, 2, 2};
-constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2, 4, 8, 4, 2};
+constexpr int kSplitVariants = 16;
+//                                       g22 g24 g42 g44 g24m2 g24m1 g28m4 g28m2 g28m1 g24m2o g28m2o g24m2h g28m2h g28m2h4 g28m2oc4 g18m2oc4
+constexpr int kSplitGA[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 1};
+constexpr int kSplitGB[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 4, 8, 4, 8, 8, 8, 8};
+constexpr int kSplitGM[kSplitVariants] = {2, 4, 2, 4, 2, 1, 4, 2, 1, 2, 2, 2, 2, 2, 2, 2};
+constexpr int kSplitGBO[kSplitVariants] = {2, 4, 2, 4, 4, 4, 8, 8, 8, 2, 2, 4, 8, 4, 2, 2};
 // y2 values interleaved into each run of the second intermediate (YBO = 2:
 // middle tiles of 2 x write runs of 4 points, last-pass tiles of 2 y2 read
 // runs of 2*GBO points)
-constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 1};
+constexpr int kSplitYBO[kSplitVariants] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1};
 // y2 values per tile of the last pass (GA unless given): c4 = tiles of 4 y2
 // reading the middle tiles' blocks of x group 2 in runs of 4*2 points
-constexpr int kSplitGAO[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 4};
+constexpr int kSplitGAO[kSplitVariants] = {2, 2, 4, 4, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 4, 4};
 constexpr int split_ga(int v) { return kSplitGA[v]; }
 constexpr int split_gao(int v) { return kSplitGAO[v]; }
-// the second intermediate with k1 inside the x group (kernels ...k): G28M2OC4,
-// whose last pass of 4 y2 values reads it faster that way
-constexpr bool split_k1o(int v) { return v == 14; }
+// the second intermediate with k1 inside the x group (kernels ...k): G28M2OC4
+// and G18M2OC4, whose last pass of 4 y2 values reads it faster that way
+constexpr bool split_k1o(int v) { return v == 14 || v == 15; }
 constexpr int split_gb(int v) { return kSplitGB[v]; }
 constexpr int split_gm(int v) { return kSplitGM[v]; }
 constexpr int split_gbo(int v) { return kSplitGBO[v]; }

@@ -141,6 +141,13 @@ static_assert(cfk::PlaneCfg<128>::THREADS == cfgeo::plane_threads(128) && cfk::P
   extern "C" __global__ __launch_bounds__(N * 4) void cf_xy1_inv_##N##_g42k(CF_ARGS) {                          \
     cfk::xy1_kernel<N, 4, 2, true, 1, true>(in, out, tw, scale, ntotal);                                       \
   }
+// the first pass of G18M2OC4: tiles of one y2 value (16 rows, 256 threads at
+// N = 256) writing the x groups of 8 of the g28 layout, two sibling tiles per
+// 128-byte line; its middle and last passes are G28M2OC4's
+#define CF_SPLIT_G18(N)                                                                                         \
+  extern "C" __global__ __launch_bounds__(N) void cf_xy1_fwd_##N##_g18(CF_ARGS) {                                \
+    cfk::xy1_kernel<N, 1, 8, false>(in, out, tw, scale, ntotal);                                                \
+  }
 // variant v of cfgeo: outer passes g<GA><GB>, middle pass g<GA><GB> (GM = GB) or g<GA><GB>m<GM>;
 // middle tiles of at most 2 x load their spectrum values with their points
 // (the registers fit; measured 2-3% faster).  Every variant at N = 128 and 256.
@@ -164,6 +171,7 @@ CF_SPLIT_H(256, 4, g24m2h)
 CF_SPLIT_H(256, 8, g28m2h)
 CF_SPLIT_H4(256)
 CF_SPLIT_OC4(256)
+CF_SPLIT_G18(256)
 CF_SPLIT_AC(128, 2, 2)
 CF_SPLIT_AC(128, 2, 4)
 CF_SPLIT_AC(128, 4, 2)
@@ -184,6 +192,7 @@ CF_SPLIT_H(128, 4, g24m2h)
 CF_SPLIT_H(128, 8, g28m2h)
 CF_SPLIT_H4(128)
 CF_SPLIT_OC4(128)
+CF_SPLIT_G18(128)
 template <int N, int V>
 struct SplitGeoCheck {
   using K = cfk::SplitGeo<N, cfgeo::split_ga(V), cfgeo::split_gb(V)>;

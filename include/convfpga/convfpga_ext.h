@@ -81,7 +81,10 @@ extern "C" {
  * of 4*2 points of the middle tiles' blocks, and those blocks ordered with k1
  * inside the x group */
 #define CF_PLAN_SPLIT3_G28M2OC4 18
-#define CF_NUM_PLANS 19
+/* G28M2OC4 with first-pass tiles of one y2 value (16 rows; 256 threads at
+ * N = 256) writing the same layout, two tiles per 128-byte line */
+#define CF_PLAN_SPLIT3_G18M2OC4 19
+#define CF_NUM_PLANS 20
 
 const char *cf_last_error(void);
 /* 1 if N is a supported cube edge, else 0 */

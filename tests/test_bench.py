@@ -100,7 +100,7 @@ def test_pass_kernel_names_of_every_four_step_plan():
     blob = open(api.CODE_OBJECT, "rb").read()
     traffic = json.load(open(os.path.join(ROOT, "profiles", "traffic_256.json")))
     plans = [name for name in api.PLAN_NAMES.values() if name.startswith("split3_")]
-    assert len(plans) == 15
+    assert len(plans) == 16
     for n in (128, 256):
         for plan in plans:
             for p in ("xy1_fwd", "y2z_fwd_mul_inv", "xy1_inv"):
@@ -110,3 +110,6 @@ def test_pass_kernel_names_of_every_four_step_plan():
                     assert k in traffic, (plan, p, k)
     assert bench.kernel_of_pass("xy1_inv", 256, "split3_g28m2oc4") == "cf_xy1_inv_256_g42k"
     assert bench.kernel_of_pass("y2z_fwd_mul_inv", 256, "split3_g28m2oc4") == "cf_mid_256_g28m2ok"
+    # one-y2 first pass; the middle and last passes of G28M2OC4
+    assert [bench.kernel_of_pass(p, 256, "split3_g18m2oc4") for p in ("xy1_fwd", "y2z_fwd_mul_inv", "xy1_inv")] == \
+        ["cf_xy1_fwd_256_g18", "cf_mid_256_g28m2ok", "cf_xy1_inv_256_g42k"]

@@ -118,12 +118,12 @@ def test_plan_support_table():
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5) == 1
         assert L.cf_plan_supported(n, api.CF_PLAN_PLANE3) == (1 if n in (32, 64, 128) else 0)
         four_step = list(range(api.CF_PLAN_SPLIT3_G22, api.CF_PLAN_SPLIT3_G28M1 + 1)) + \
-            list(range(api.CF_PLAN_SPLIT3_G24M2O, api.CF_PLAN_SPLIT3_G28M2OC4 + 1))
+            list(range(api.CF_PLAN_SPLIT3_G24M2O, api.CF_PLAN_SPLIT3_G18M2OC4 + 1))
         for p in four_step:
             want = n in (128, 256)
             assert L.cf_plan_supported(n, p) == (1 if want else 0)
         assert L.cf_plan_supported(n, api.CF_PLAN_AXIS5Z) == (1 if n >= 64 else 0)
-    assert api.CF_NUM_PLANS == api.CF_PLAN_SPLIT3_G28M2OC4 + 1 == len(api.PLAN_NAMES) + 1
+    assert api.CF_NUM_PLANS == api.CF_PLAN_SPLIT3_G18M2OC4 + 1 == len(api.PLAN_NAMES) + 1
     assert L.cf_plan_supported(1024, api.CF_PLAN_AXIS5) == 0
     assert L.cf_plan_supported(256, 99) == 0
     assert L.cf_set_plan(256, 99) == api.CF_ERR_ARG

@@ -290,7 +290,7 @@ def test_device_fill_matches_host_generator(gpu, oracle):
 
 PLAN_CASES = [(32, 1), (32, 2), (64, 1), (64, 2), (128, 1), (128, 2), (256, 1)] + \
     [(256, p) for p in range(3, 12)] + [(512, 1)] + [(n, 12) for n in (64, 128, 256, 512)] + [(128, p) for p in range(3, 12)] + \
-    [(n, p) for n in (128, 256) for p in (13, 14, 15, 16, 17, 18)]
+    [(n, p) for n in (128, 256) for p in (13, 14, 15, 16, 17, 18, 19)]
 
 
 @pytest.mark.parametrize("n,plan", PLAN_CASES)
