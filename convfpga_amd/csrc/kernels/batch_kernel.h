@@ -163,14 +163,23 @@ __device__ __forceinline__ void batch_xy1_tile_body(const BatchTile& t) {
   constexpr int GB = INV ? B::GBO : B::GB, YB = INV ? B::YBO : 1;
   using K = SplitGeo<N, 2, GB, YB>;
   using P = Xy1Pass<N, 2, GB, INV, YB>;
-  const int x = t.tid, row = x / K::T;
-  const int cxl = YB == 1 ? x % GB : (x / YB) % GB;
-  const int cy2l = YB == 1 ? (x / GB) % 2 : x % YB;
-  const int cxh = x / (GB * 2);
-  const P k{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, false};
+  // The pass object is built for each phase from a thread id the compiler
+  // cannot carry across them, so that the load's offsets are dead once the
+  // loads are issued.  Built once, its members stay live through the tile
+  // and spill at the 128-VGPR cap; the reloads then sit between the tile's
+  // loads, each waiting on the in-order vector memory counter.
+  auto make = [&]() {
+    int x = t.tid;
+    asm volatile("" : "+v"(x));
+    const int row = x / K::T;
+    const int cxl = YB == 1 ? x % GB : (x / YB) % GB;
+    const int cy2l = YB == 1 ? (x / GB) % 2 : x % YB;
+    const int cxh = x / (GB * 2);
+    return P{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, false};
+  };
   typename P::Regs r;
-  k.load(t.tile, r);
-  k.process(t.tile, r);
+  make().load(t.tile, r);
+  make().process(t.tile, r);
 }
 
 template <int N, int V>
@@ -178,10 +187,14 @@ __device__ __forceinline__ void batch_mid_tile_body(const BatchTile& t) {
   using B = BatchSplit<V>;
   using M = MidGeoN<N, B::GB, 2>;
   using P = MidPass<N, 2, B::GB, 2, false, B::GBO, B::YBO>;
-  const P k{t.in, t.out, t.H, t.tw_s, t.lds, 0, t.tid % M::LINES, t.tid / M::LINES, t.tid, false};
+  auto make = [&]() {  // per phase, as in batch_xy1_tile_body
+    int x = t.tid;
+    asm volatile("" : "+v"(x));
+    return P{t.in, t.out, t.H, t.tw_s, t.lds, 0, x % M::LINES, x / M::LINES, x, false};
+  };
   typename P::Regs r;
-  k.load(t.tile, r);
-  k.process(t.tile, r);
+  make().load(t.tile, r);
+  make().process(t.tile, r);
 }
 
 template <int N>
