@@ -14,7 +14,11 @@ struct CfBatchSeg {
   float scale;      // final scale (last inverse pass)
   int n;            // cube edge
   int kind;         // CfBatchKind
-  int variant;      // four-step tile variant (cfgeo::split_*), 0 otherwise
+  short variant;    // four-step tile variant (cfgeo::split_*), 0 otherwise
+  short stream;     // 1: the pass reads its signal and spectrum and writes the
+                    // convolution's output non-temporally (a 256^3 instance,
+                    // larger than the memory-side cache holds beside its
+                    // intermediates, whose spectrum no other instance reads)
   int first_item;   // queue index of the segment's first work item
   int ntiles;       // tiles of the pass
   int per_item;     // tiles per work item (a multiple of the tiles a work-group runs at once)

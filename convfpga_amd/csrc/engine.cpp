@@ -1949,6 +1949,20 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
     int stage_need[kBatchMaxStages] = {};
     int stage_first[kBatchMaxStages + 1] = {};  // first work item of each stage (stage-major numbering)
     int item = 0;
+    // 256^3 instances whose spectrum no other instance reads stream their
+    // signal, spectrum and output past the memory-side cache (256 MB), which
+    // then keeps more of their intermediates, as the single-cube passes do;
+    // smaller cubes fit it and lose by it (measured)
+    std::vector<short> stream_of(count, 0);
+    {
+      std::vector<const void*> hs(count);
+      for (unsigned i = 0; i < count; ++i) hs[i] = inst[i].h->d;
+      std::sort(hs.begin(), hs.end());
+      for (unsigned i = 0; i < count; ++i) {
+        const auto r = std::equal_range(hs.begin(), hs.end(), (const void*)inst[i].h->d);
+        stream_of[i] = (short)(inst[i].h->n == 256 && r.second - r.first == 1);
+      }
+    }
     // within a stage, the segments by edge, largest first: a work-group's
     // items (G apart) then change edge, and restage its twiddles, rarely
     std::vector<unsigned> by_edge(count);
@@ -1968,7 +1982,8 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
         sg.scale = q.scale;
         sg.n = (int)inst[i].h->n;
         sg.kind = q.kind;
-        sg.variant = q.variant;
+        sg.variant = (short)q.variant;
+        sg.stream = stream_of[i];
         sg.first_item = item;
         sg.ntiles = q.ntiles;
         // A work item: the tiles a work-group runs side by side.  Sibling tiles

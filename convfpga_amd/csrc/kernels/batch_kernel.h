@@ -101,6 +101,7 @@ struct BatchTile {
   int tid;   // thread within the tile's group
   float2* lds;  // the group's LDS slice
   const float2* tw_s;
+  bool nt;      // non-temporal signal, spectrum and output (CfBatchSeg::stream)
 };
 
 // The tile bodies.  The general kernel calls them out of line, one function
@@ -175,7 +176,7 @@ __device__ __forceinline__ void batch_xy1_tile_body(const BatchTile& t) {
     const int cxl = YB == 1 ? x % GB : (x / YB) % GB;
     const int cy2l = YB == 1 ? (x / GB) % 2 : x % YB;
     const int cxh = x / (GB * 2);
-    return P{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, false};
+    return P{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, row, row % 16, row / 16, cxh * GB + cxl, cy2l, t.nt};
   };
   typename P::Regs r;
   make().load(t.tile, r);
@@ -190,7 +191,7 @@ __device__ __forceinline__ void batch_mid_tile_body(const BatchTile& t) {
   auto make = [&]() {  // per phase, as in batch_xy1_tile_body
     int x = t.tid;
     asm volatile("" : "+v"(x));
-    return P{t.in, t.out, t.H, t.tw_s, t.lds, 0, x % M::LINES, x / M::LINES, x, false};
+    return P{t.in, t.out, t.H, t.tw_s, t.lds, 0, x % M::LINES, x / M::LINES, x, t.nt};
   };
   typename P::Regs r;
   make().load(t.tile, r);
@@ -300,7 +301,7 @@ __device__ __forceinline__ void batch_item(const CfBatchSeg& sg, int t0, int t1,
   // buffer descriptors without waterfall loops
   const int sub = kBatchThreads / tt, g = __builtin_amdgcn_readfirstlane(tid / tt);
   BatchTile t{(const float2*)sg.in, (float2*)sg.out, (const float2*)sg.H, sg.scale, 0, tid % tt,
-              lds + g * batch_tile_lds<N>(sg.kind), tw_s};
+              lds + g * batch_tile_lds<N>(sg.kind), tw_s, sg.stream != 0};
   for (int tile = t0; tile < t1; tile += sub) {
     // the tiles before may still use the LDS: the pass code ends with exchanges
     // that only order a wave's own accesses (row FFTs), so a barrier separates
