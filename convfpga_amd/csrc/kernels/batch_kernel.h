@@ -135,10 +135,14 @@ __device__ __forceinline__ void batch_axis_tile_body(int kind, const BatchTile& 
         k.load(t.tile, r);
         k.process(t.tile, r);
       } else if (kind == CF_BK_COLZ_CONV) {
-        const ColPass<N, 2, 2> k{t.in, t.out, t.H, t.tw_s, t.lds, 0, c, tt, g};
+        auto make = [&]() {  // per phase, as in batch_xy1_tile_body
+          int x = t.tid;
+          asm volatile("" : "+v"(x));
+          return ColPass<N, 2, 2>{t.in, t.out, t.H, t.tw_s, t.lds, 0, x % K::C, (x / K::C) % K::T, x / (K::C * K::T)};
+        };
         typename ColPass<N, 2, 2>::Regs r;
-        k.load(t.tile, r);
-        k.process(t.tile, r);
+        make().load(t.tile, r);
+        make().process(t.tile, r);
       } else {
         const ColPass<N, 1, 1> k{t.in, t.out, t.H, t.tw_s, t.lds, 0, c, tt, g};
         typename ColPass<N, 1, 1>::Regs r;
@@ -152,10 +156,14 @@ __device__ __forceinline__ void batch_axis_tile_body(int kind, const BatchTile& 
 template <int N, bool INV>
 __device__ __forceinline__ void batch_plane_tile_body(const BatchTile& t) {
   using K = PlaneCfg<N>;
-  const PlanePass<N, INV> k{t.in, t.out, t.tw_s, t.lds, t.scale, t.tid % K::T, t.tid / K::T, t.tid % N, t.tid / N};
+  auto make = [&]() {  // per phase, as in batch_xy1_tile_body
+    int x = t.tid;
+    asm volatile("" : "+v"(x));
+    return PlanePass<N, INV>{t.in, t.out, t.tw_s, t.lds, t.scale, x % K::T, x / K::T, x % N, x / N};
+  };
   typename PlanePass<N, INV>::Regs r;
-  k.load(t.tile, r);
-  k.process(t.tile, r);
+  make().load(t.tile, r);
+  make().process(t.tile, r);
 }
 
 template <int N, int V, bool INV>
