@@ -43,6 +43,9 @@ enum CfBatchKind : int {
 };
 
 constexpr int kBatchThreads = 512;  // threads of a work-group of the persistent kernel
+// dynamic LDS (bytes, unused) reserved by launches of the small-edge form so
+// that two of its work-groups, not three, share a CU's 160 KiB
+constexpr unsigned kBatchSmallReserve = 32768;
 constexpr int kBatchMaxStages = 5;  // passes of the longest schedule
 constexpr int kBatchXcds = 16;      // XCD ids the counters provide for (MI355X has 8)
 // counters: [0] work-groups started, [1] error flag, [2] work-groups

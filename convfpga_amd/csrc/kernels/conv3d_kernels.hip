@@ -268,9 +268,9 @@ extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_wav
     unsigned long long spin_limit) {
   cfk::batch_kernel<false>(segs, stage_need, nitems, ctr, spin_limit);
 }
-// the same for the plane plan at edges of 32 and 64 only: half the LDS, three
-// work-groups per CU (four, at 64 VGPRs and 2 KiB less of item list, measured
-// slower)
+// the same for the plane plan at edges of 32 and 64 only: half the LDS and
+// 45 VGPRs; its launches reserve LDS so that two work-groups share a CU
+// (three and four measured slower)
 extern "C" __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) void cf_batch_run_3p_small(
     const CfBatchSeg* segs, const int* stage_need, int nitems, unsigned* ctr,
     unsigned long long spin_limit) {
