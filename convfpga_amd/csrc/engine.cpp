@@ -2055,7 +2055,12 @@ static int batch_create(const cf_instance_t* inst, unsigned count, int scale_mod
       batch_free(b);
       return fail(CF_ERR_NOMEM, "batch descriptor allocation: %s", hipGetErrorString(e));
     }
-    const unsigned grid = (unsigned)std::min(nitems, full_grid);
+    // as many work-groups as the largest stage has items (rounded up to whole
+    // groups of the 8 XCDs): more would sit idle in every stage yet take part
+    // in its counting (one 128^3 convolution: 0.066 against 0.078 ms)
+    int most = 1;
+    for (int st = 0; st < nstages; ++st) most = std::max(most, stage_first[st + 1] - stage_first[st]);
+    const unsigned grid = (unsigned)std::min(full_grid, (most + 7) / 8 * 8);
     HIPCHK(hipStreamBeginCapture(g.capture, hipStreamCaptureModeRelaxed));
     unsigned* ctr = b->d_ctr;
     const void* sp = b->d_segs;
